@@ -1,0 +1,300 @@
+/* cu_fp_core.cuh -- word-level engine behind cu_fp::cu_freal<PB> / cu_fcomplex<PB>
+ * for AMD CDNA4 (gfx950) and the host.
+ *
+ * Significands are handled as little-endian arrays of 32-bit words so that every
+ * word product maps onto one v_mad_u64_u32 and every carry onto a v_add_co /
+ * v_addc_co pair.  All loops have compile-time trip counts and every
+ * data-dependent shift is a select network (log-step barrel shifter over words
+ * plus a funnel shift), so the arrays stay in VGPRs and lanes of a wavefront do
+ * not diverge on exponent differences.  The only branch kept is the word-level
+ * normalisation after a cancellation of 32 bits or more, which random data
+ * almost never takes.
+ *
+ * Rounding is round-to-nearest-even (MPFR RNDN).  The algorithm is the one of
+ * the reference's single rounding kernel (tkouya/mpc_cuda
+ * include/mpc_cuda/cu_freal.cuh:89-138): keep PB bits below the most
+ * significant one, a round bit, and a sticky bit for everything lower.
+ */
+#ifndef MPGFX_CU_FP_CORE_CUH
+#define MPGFX_CU_FP_CORE_CUH
+
+#include <cstdint>
+#include <type_traits>
+#if defined(__HIPCC__) || defined(__HIP__)
+#include <hip/hip_runtime.h>
+#define CUFP_INL __host__ __device__ __forceinline__
+#else
+#ifndef __host__
+#define __host__
+#endif
+#ifndef __device__
+#define __device__
+#endif
+#define CUFP_INL inline __attribute__ ((always_inline))
+#endif
+
+namespace cu_fp {
+namespace core {
+
+typedef uint32_t u32;
+typedef uint64_t u64;
+typedef int64_t i64;
+
+#define CUFP_UNROLL _Pragma ("unroll")
+
+/* compile-time loop: f(std::integral_constant<int, I>) for I in [B, E); keeps
+ * every array index a constant so register arrays never fall back to scratch */
+template <int B, int E, typename F>
+CUFP_INL void sfor (F &&f)
+{
+  if constexpr (B < E)
+    {
+      f (std::integral_constant<int, B> ());
+      sfor<B + 1, E> (f);
+    }
+}
+
+/* s = a + b + c, c <- carry out (c in {0,1}) */
+CUFP_INL u32 addc (u32 a, u32 b, u32 &c)
+{
+  u64 t = (u64) a + b + c;
+  c = (u32) (t >> 32);
+  return (u32) t;
+}
+
+/* top 32 bits of (hi:lo) << s, s in [0,31] */
+CUFP_INL u32 fshl (u32 hi, u32 lo, u32 s)
+{
+  return (u32) (((((u64) hi << 32) | lo) << s) >> 32);
+}
+/* low 32 bits of (hi:lo) >> s, s in [0,31] */
+CUFP_INL u32 fshr (u32 hi, u32 lo, u32 s)
+{
+  return (u32) ((((u64) hi << 32) | lo) >> s);
+}
+
+CUFP_INL int clz32 (u32 x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return x ? __builtin_clz (x) : 32;
+#else
+  return x ? __builtin_clz (x) : 32;
+#endif
+}
+
+/* ---------------------------------------------------------------------------
+ * Exact product: p[2W] = a[W] * b[W] (schoolbook by rows; each step is one
+ * 32x32+64 -> 64 multiply-add).
+ * ------------------------------------------------------------------------- */
+template <int W>
+CUFP_INL void mul_words (u32 (&p)[2 * W], const u32 (&a)[W], const u32 (&b)[W])
+{
+  CUFP_UNROLL
+  for (int i = 0; i < 2 * W; i++) p[i] = 0;
+  sfor<0, W> ([&] (auto J) {
+    constexpr int j = decltype (J)::value;
+    u32 c = 0;
+    sfor<0, W> ([&] (auto I) {
+      constexpr int i = decltype (I)::value;
+      u64 t = (u64) a[i] * b[j] + p[i + j] + c;
+      p[i + j] = (u32) t;
+      c = (u32) (t >> 32);
+    });
+    p[W + j] = c;
+  });
+}
+
+/* ---------------------------------------------------------------------------
+ * Round the LEN-word buffer D, whose top bit is bit 32*LEN-1 after a left shift
+ * by s in {0,1}, to KW = PB/32 words (RNDN).  `xst` is sticky material below
+ * D bit 0.  Returns the kept words in K and 1 on carry-out (K is then
+ * 2^(PB-1) and the caller adds 1 to the exponent).
+ * ------------------------------------------------------------------------- */
+template <int KW, int LEN>
+CUFP_INL u32 round_top (u32 (&K)[KW], const u32 (&D)[LEN], u32 s, u32 xst)
+{
+  static_assert (LEN >= KW, "buffer narrower than the precision");
+  /* kept words: shifted words LEN-KW .. LEN-1 */
+  CUFP_UNROLL
+  for (int j = 0; j < KW; j++)
+    {
+      int t = LEN - KW + j;
+      u32 lo = (t >= 1) ? D[t - 1] : 0u;
+      K[j] = fshl (D[t], lo, s);
+    }
+  u32 rbit = 0, sticky = xst;
+  if (LEN > KW)
+    {
+      const int r = LEN - KW - 1;
+      u32 lo = (r >= 1) ? D[r - 1] : 0u;
+      u32 rw = fshl (D[r], lo, s);
+      rbit = rw >> 31;
+      sticky |= rw & 0x7fffffffu;
+      CUFP_UNROLL
+      for (int i = 0; i < r; i++) sticky |= D[i];
+    }
+  u32 inc = rbit & ((sticky != 0) | (K[0] & 1u));
+  u32 c = inc;
+  CUFP_UNROLL
+  for (int j = 0; j < KW; j++) K[j] = addc (K[j], 0u, c);
+  /* carry-out: K wrapped to 0; the value is 2^PB = 2^(PB-1) * 2 */
+  K[KW - 1] |= c << 31;
+  return c;
+}
+
+/* ---------------------------------------------------------------------------
+ * RNDN(A +/- B) for two exact operands of LW words each (top bit set unless
+ * zero), value = sign * M * 2^(e - 32*LW), rounded to KW words.  The larger
+ * magnitude is placed in a window of LW+2 words (one guard word below, one
+ * carry word above); the smaller is shifted right by the exponent difference
+ * with a barrel shifter; bits that leave the window become the sticky bit and,
+ * for a subtraction, a borrow of one unit at the window's LSB (the exact floor
+ * of the true difference -- cu_freal.cuh:266-270 uses the same device).
+ * Output: sign, exponent, zero flag and the kept words.
+ * ------------------------------------------------------------------------- */
+template <int KW>
+struct rres
+{
+  int sign;
+  i64 exp;
+  bool zero;
+  u32 k[KW];
+};
+
+template <int LW>
+CUFP_INL bool mag_ge (i64 ea, const u32 (&A)[LW], i64 eb, const u32 (&B)[LW])
+{
+  /* lexicographic (exp, words from the top); a select chain, no branches */
+  int c = 0;                                 /* result of word compare: -1,0,1 */
+  CUFP_UNROLL
+  for (int i = 0; i < LW; i++)
+    {
+      int ci = (A[i] > B[i]) ? 1 : ((A[i] < B[i]) ? -1 : 0);
+      c = ci ? ci : c;                         /* later (higher) words win */
+    }
+  return (ea != eb) ? (ea > eb) : (c >= 0);
+}
+
+template <int KW, int LW>
+CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
+                             int sb, i64 eb, bool zb, const u32 (&Bi)[LW])
+{
+  const int L = LW + 2;
+  rres<KW> r;
+  /* order by magnitude: zero is the smallest */
+  bool age = zb || (!za && mag_ge<LW> (ea, Ai, eb, Bi));
+  u32 A[LW], B[LW];
+  CUFP_UNROLL
+  for (int i = 0; i < LW; i++)
+    {
+      A[i] = age ? Ai[i] : Bi[i];
+      B[i] = age ? Bi[i] : Ai[i];
+    }
+  int sA = age ? sa : sb, sB = age ? sb : sa;
+  i64 eA = age ? ea : eb, eB = age ? eb : ea;
+  bool zA = age ? za : zb, zB = age ? zb : za;
+  const i64 dmax = 32 * (i64) (LW + 1);
+  i64 d = zB ? dmax : (eA - eB);
+  d = d > dmax ? dmax : d;
+  u32 ws = (u32) (d >> 5), bs = (u32) (d & 31);
+
+  /* T = [guard 0][B][carry 0] shifted right by d */
+  u32 T[L];
+  T[0] = 0;
+  CUFP_UNROLL
+  for (int i = 0; i < LW; i++) T[i + 1] = B[i];
+  T[L - 1] = 0;
+  u32 ext = 0;
+  CUFP_UNROLL
+  for (int k = 0; (1 << k) <= LW + 1; k++)
+    {
+      const int sh = 1 << k;
+      u32 on = (ws >> k) & 1u;
+      u32 lost = 0;
+      CUFP_UNROLL
+      for (int i = 0; i < sh && i < L; i++) lost |= T[i];
+      ext |= on ? lost : 0u;
+      CUFP_UNROLL
+      for (int i = 0; i < L; i++)
+        {
+          u32 v = (i + sh < L) ? T[i + sh] : 0u;
+          T[i] = on ? v : T[i];
+        }
+    }
+  {
+    u32 mask = bs ? ((1u << bs) - 1u) : 0u;
+    ext |= T[0] & mask;
+    CUFP_UNROLL
+    for (int i = 0; i < L; i++)
+      {
+        u32 hi = (i + 1 < L) ? T[i + 1] : 0u;
+        T[i] = fshr (hi, T[i], bs);
+      }
+  }
+  u32 st = (ext != 0);
+
+  /* D = A_win + (sub ? ~T + 1 - st : T) */
+  bool sub = (sA != sB) && !zB;
+  u32 xm = sub ? 0xffffffffu : 0u;
+  u32 c = (sub && !st) ? 1u : 0u;
+  u32 D[L];
+  D[0] = addc (0u, T[0] ^ xm, c);
+  CUFP_UNROLL
+  for (int i = 0; i < LW; i++) D[i + 1] = addc (A[i], T[i + 1] ^ xm, c);
+  D[L - 1] = addc (0u, T[L - 1] ^ xm, c);
+
+  /* carry into the top word: shift right by one, dropped bit -> sticky */
+  u32 cy = D[L - 1] & 1u;
+  st |= cy & D[0];
+  u32 E[L - 1];
+  CUFP_UNROLL
+  for (int i = 0; i < L - 1; i++)
+    {
+      u32 hi = D[i + 1];
+      E[i] = cy ? fshr (hi, D[i], 1u) : D[i];
+    }
+  /* leading zeros of E (L-1 words) */
+  u32 lz = 0;
+  bool seen = false;
+  CUFP_UNROLL
+  for (int i = L - 2; i >= 0; i--)
+    {
+      u32 z = (u32) clz32 (E[i]);
+      lz += seen ? 0u : z;
+      seen = seen || (E[i] != 0);
+    }
+  bool zres = !seen;
+  u32 wl = lz >> 5, bl = lz & 31u;
+  if (wl != 0)
+    {
+      /* cancellation of >= 32 bits: word-level left shift (rare) */
+      CUFP_UNROLL
+      for (int k = 0; (1 << k) <= L - 1; k++)
+        {
+          const int sh = 1 << k;
+          u32 on = (wl >> k) & 1u;
+          CUFP_UNROLL
+          for (int i = L - 2; i >= 0; i--)
+            {
+              u32 v = (i - sh >= 0) ? E[i - sh] : 0u;
+              E[i] = on ? v : E[i];
+            }
+        }
+    }
+  CUFP_UNROLL
+  for (int i = L - 2; i >= 0; i--)
+    {
+      u32 lo = (i >= 1) ? E[i - 1] : 0u;
+      E[i] = fshl (E[i], lo, bl);
+    }
+  u32 co = round_top<KW, L - 1> (r.k, E, 0u, st);
+  r.sign = zres ? 1 : sA;
+  r.exp = eA + (i64) cy - (i64) lz + (i64) co;
+  r.zero = zres;
+  return r;
+}
+
+} // namespace core
+} // namespace cu_fp
+
+#endif

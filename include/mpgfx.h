@@ -1,0 +1,152 @@
+/* mpgfx.h -- C ABI of the MI355X (gfx950) fixed-precision multiple-precision
+ * library: batched RNDN arithmetic, AXPY, GEMV and GEMM on cu_fp::cu_freal<PB>
+ * / cu_fcomplex<PB> values held in device memory.
+ *
+ * The reference (tkouya/mpc_cuda) exposes this path only as a header-only C++
+ * template API plus example __global__ kernels; each entry point below names
+ * the reference kernel or function whose behaviour it provides.  The template
+ * API itself is available unchanged in include/mpc_cuda/cu_freal.cuh and
+ * include/mpc_cuda/cu_fcomplex.cuh.
+ *
+ * Conventions
+ *  - All array pointers are DEVICE pointers owned by the caller.  `stream` is a
+ *    hipStream_t passed as void* (NULL = the default stream).  Every call is
+ *    stream-ordered and asynchronous; it returns after enqueueing work.
+ *  - Precision pb is in bits, a multiple of 32; mpgfx_supported_pb() tells
+ *    which precisions are compiled in.
+ *  - Value arrays are structure-of-arrays ("MP arrays").  A real MP array of
+ *    n values at precision pb is
+ *        limbs : N = ceil(pb/64) planes of n uint64; limb l of value i is
+ *                limbs[l*n + i] (little-endian limbs, MPFR-compatible,
+ *                bit 63 of limb N-1 set, the low 64N-pb bits zero)
+ *        se    : n int64 sign/exponent words, se = exp*2 + (sign<0 ? 1 : 0);
+ *                exp is the MPFR exponent, value = sign*M*2^(exp-64N); a zero
+ *                value has exp == MPGFX_ZERO_EXP and all limbs zero.
+ *    A complex MP array of n values is the real MP array of the n real parts
+ *    followed by the real MP array of the n imaginary parts: limbs has 2*N*n
+ *    entries (re planes, then im planes), se has 2*n entries (re, then im).
+ *  - Matrices are MP arrays in row-major element order: element (i, j) of an
+ *    m x n matrix is value i*n + j of an MP array of m*n values.
+ *  - Results are bit-exact with MPFR (RNDN) / MPC (MPC_RNDNN) for the same
+ *    operation order; zeros follow the reference's conventions (exact
+ *    cancellation and products with a zero factor give +0).
+ *  - Return: MPGFX_OK, or an error code; nothing is launched on error.
+ */
+#ifndef MPGFX_H
+#define MPGFX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPGFX_ZERO_EXP (-0x7fffffffffffffLL)
+
+enum mpgfx_status {
+  MPGFX_OK = 0,
+  MPGFX_EINVAL = 1,        /* bad size, null pointer, bad mode */
+  MPGFX_EUNSUPPORTED = 2,  /* precision not compiled in */
+  MPGFX_EHIP = 3,          /* HIP launch or runtime error */
+  MPGFX_ENOMEM = 4         /* workspace allocation failed */
+};
+
+enum mpgfx_op { MPGFX_MUL = 0, MPGFX_ADD = 1, MPGFX_SUB = 2 };
+enum mpgfx_gemm_mode { MPGFX_GEMM_PEROP = 0, MPGFX_GEMM_EXACT = 1 };
+
+/* Library version (major*10000 + minor*100 + patch). */
+int mpgfx_version (void);
+
+/* 1 if precision pb is compiled into the elementwise / AXPY / GEMV / GEMM
+ * kernels, else 0. */
+int mpgfx_supported_pb (int pb);
+
+/* Text of the last HIP error seen by this thread (empty if none). */
+const char *mpgfx_last_error (void);
+
+/* o[i] = a[i] OP b[i] for real MP arrays of n values, OP in {mul, add, sub},
+ * each RNDN.  Replaces the reference's per-element use of cu_fmul / cu_fadd /
+ * cu_fsub: tools/test_freal.cu:41-48 (kern<PB>), cu_freal.cuh:203-213, 280-292. */
+int mpgfx_elementwise_real (int pb, int op, int64_t n,
+                            const uint64_t *a_limbs, const int64_t *a_se,
+                            const uint64_t *b_limbs, const int64_t *b_se,
+                            uint64_t *o_limbs, int64_t *o_se, void *stream);
+
+/* Complex counterpart, MPC_RNDNN per component: cu_cmul / cu_cadd / cu_csub
+ * applied elementwise; replaces tools/test_fcomplex.cu:147-153 (kern<PB>),
+ * cu_fcomplex.cuh:115-124. */
+int mpgfx_elementwise_complex (int pb, int op, int64_t n,
+                               const uint64_t *a_limbs, const int64_t *a_se,
+                               const uint64_t *b_limbs, const int64_t *b_se,
+                               uint64_t *o_limbs, int64_t *o_se, void *stream);
+
+/* o[i] = RNDN(RNDN(a*x[i]) + y[i]); a is a real MP array of ONE value.
+ * Replaces demos/bench_3way.cu:44-50 (freal_axpy<P>) and
+ * demos/sample_fixed.cu:20-30 (axpy<PB>) on full-limb values. */
+int mpgfx_axpy_real (int pb, int64_t n,
+                     const uint64_t *a_limbs, const int64_t *a_se,
+                     const uint64_t *x_limbs, const int64_t *x_se,
+                     const uint64_t *y_limbs, const int64_t *y_se,
+                     uint64_t *o_limbs, int64_t *o_se, void *stream);
+
+/* z[i] = a*x[i] + y[i] for complex MP arrays (a: ONE complex value), the
+ * product and the sum each MPC_RNDNN.  Replaces demos/bench_3way.cu:51-59
+ * (fcplx_axpy<P>) and tools/axpy_fast_complex.cu:32-40 (axpyc). */
+int mpgfx_axpy_complex (int pb, int64_t n,
+                        const uint64_t *a_limbs, const int64_t *a_se,
+                        const uint64_t *x_limbs, const int64_t *x_se,
+                        const uint64_t *y_limbs, const int64_t *y_se,
+                        uint64_t *z_limbs, int64_t *z_se, void *stream);
+
+/* y[i] = fold over k = 0..k-1 of acc = RNDN(acc + RNDN(A[i,k]*x[k])), acc
+ * starting at +0 (per-operation rounding in sequential k order).  A is an
+ * m x k real MP matrix, x a real MP array of k, y a real MP array of m.
+ * Replaces tools/linalg_fast.cu:37-51 (matvec_kernel). */
+int mpgfx_gemv_real (int pb, int64_t m, int64_t k,
+                     const uint64_t *A_limbs, const int64_t *A_se,
+                     const uint64_t *x_limbs, const int64_t *x_se,
+                     uint64_t *y_limbs, int64_t *y_se, void *stream);
+
+/* Bytes of device workspace mpgfx_gemm_real needs for this shape (0 for the
+ * per-operation mode).  No counterpart in the reference (its kernels need no
+ * workspace). */
+int mpgfx_gemm_workspace_size (int pb, int mode, int64_t m, int64_t n, int64_t k,
+                               size_t *bytes);
+
+/* C = A*B, A m x k, B k x n, C m x n (real MP matrices).
+ *  mode MPGFX_GEMM_PEROP: C[i,j] = fold over k of RNDN(acc + RNDN(A[i,k]*B[k,j])),
+ *    acc from +0, k ascending -- the reference's matmul, tools/linalg_fast.cu:53-68
+ *    (matmul_kernel).
+ *  mode MPGFX_GEMM_EXACT: C[i,j] = RNDN(sum_k A[i,k]*B[k,j]) with the sum exact
+ *    and ONE rounding (the semantics of mpfr_dot: exact products then
+ *    mpfr_sum, src/mpfr_cuda/dot.c:47-54); slice products run on INT8 MFMA.
+ * `ws`/`ws_bytes`: caller workspace of at least mpgfx_gemm_workspace_size()
+ * bytes; with ws == NULL the library allocates it stream-ordered itself. */
+int mpgfx_gemm_real (int pb, int mode, int64_t m, int64_t n, int64_t k,
+                     const uint64_t *A_limbs, const int64_t *A_se,
+                     const uint64_t *B_limbs, const int64_t *B_se,
+                     uint64_t *C_limbs, int64_t *C_se,
+                     void *ws, size_t ws_bytes, void *stream);
+
+/* Complex per-operation GEMM (C = A*B, acc = +0, acc = acc + A[i,k]*B[k,j] with
+ * MPC_RNDNN products and sums, k ascending).  Counterpart of the reference's
+ * complex matmul demo order (demos/matmul_mpc.cu:33-52) on the fixed type. */
+int mpgfx_gemm_complex (int pb, int64_t m, int64_t n, int64_t k,
+                        const uint64_t *A_limbs, const int64_t *A_se,
+                        const uint64_t *B_limbs, const int64_t *B_se,
+                        uint64_t *C_limbs, int64_t *C_se, void *stream);
+
+/* Convert between arrays of the reference's AoS structs (cu_freal<PB>:
+ * {int sign; long exp; cu_limb m[N];}, sizeof 16+8N, cu_freal.cuh:75-77;
+ * cu_fcomplex<PB>: {re, im}, cu_fcomplex.cuh:102) and MP arrays. `aos` is a
+ * device pointer to n structs; `complex` selects cu_fcomplex<PB>. */
+int mpgfx_pack (int pb, int complex, int64_t n, const void *aos,
+                uint64_t *limbs, int64_t *se, void *stream);
+int mpgfx_unpack (int pb, int complex, int64_t n, const uint64_t *limbs,
+                  const int64_t *se, void *aos, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,153 @@
+/* api.hip -- the C ABI of the mpgfx library (declared in include/mpgfx.h):
+ * argument checking, precision dispatch and error bookkeeping.  Every entry
+ * point validates its arguments on the host and launches nothing on error. */
+#include "common.h"
+#include "dispatch.h"
+#include <cstring>
+
+namespace mpgfx {
+
+static thread_local char g_err[256];
+
+void set_error (const char *where, hipError_t e)
+{
+  snprintf (g_err, sizeof g_err, "%s: %s", where, hipGetErrorString (e));
+}
+
+int launch_status (const char *where)
+{
+  hipError_t e = hipGetLastError ();
+  if (e != hipSuccess) { set_error (where, e); return MPGFX_EHIP; }
+  return MPGFX_OK;
+}
+
+const pb_ops *find_ops (int pb)
+{
+#define C(P) if (pb == P) return ops_pb_##P ();
+  MPGFX_PB_LIST (C)
+#undef C
+  return nullptr;
+}
+
+} // namespace mpgfx
+
+using namespace mpgfx;
+
+#define NEED(p) do { if (!(p)) return MPGFX_EINVAL; } while (0)
+#define GET_OPS(o) const pb_ops *o = find_ops (pb); if (!o) return MPGFX_EUNSUPPORTED
+
+extern "C" int mpgfx_version (void) { return 100; }
+
+extern "C" int mpgfx_supported_pb (int pb) { return find_ops (pb) != nullptr; }
+
+extern "C" const char *mpgfx_last_error (void) { return g_err; }
+
+extern "C" int mpgfx_elementwise_real (int pb, int op, int64_t n, const uint64_t *al, const int64_t *ase,
+                                       const uint64_t *bl, const int64_t *bse, uint64_t *ol, int64_t *ose,
+                                       void *stream)
+{
+  if (n < 0 || op < MPGFX_MUL || op > MPGFX_SUB) return MPGFX_EINVAL;
+  GET_OPS (o);
+  if (n == 0) return MPGFX_OK;
+  NEED (al && ase && bl && bse && ol && ose);
+  return o->ew_real (op, n, al, ase, bl, bse, ol, ose, (hipStream_t) stream);
+}
+
+extern "C" int mpgfx_elementwise_complex (int pb, int op, int64_t n, const uint64_t *al, const int64_t *ase,
+                                          const uint64_t *bl, const int64_t *bse, uint64_t *ol, int64_t *ose,
+                                          void *stream)
+{
+  if (n < 0 || op < MPGFX_MUL || op > MPGFX_SUB) return MPGFX_EINVAL;
+  GET_OPS (o);
+  if (n == 0) return MPGFX_OK;
+  NEED (al && ase && bl && bse && ol && ose);
+  return o->ew_cplx (op, n, al, ase, bl, bse, ol, ose, (hipStream_t) stream);
+}
+
+extern "C" int mpgfx_axpy_real (int pb, int64_t n, const uint64_t *al, const int64_t *ase, const uint64_t *xl,
+                                const int64_t *xse, const uint64_t *yl, const int64_t *yse, uint64_t *ol,
+                                int64_t *ose, void *stream)
+{
+  if (n < 0) return MPGFX_EINVAL;
+  GET_OPS (o);
+  if (n == 0) return MPGFX_OK;
+  NEED (al && ase && xl && xse && yl && yse && ol && ose);
+  return o->axpy_real (n, al, ase, xl, xse, yl, yse, ol, ose, (hipStream_t) stream);
+}
+
+extern "C" int mpgfx_axpy_complex (int pb, int64_t n, const uint64_t *al, const int64_t *ase, const uint64_t *xl,
+                                   const int64_t *xse, const uint64_t *yl, const int64_t *yse, uint64_t *zl,
+                                   int64_t *zse, void *stream)
+{
+  if (n < 0) return MPGFX_EINVAL;
+  GET_OPS (o);
+  if (n == 0) return MPGFX_OK;
+  NEED (al && ase && xl && xse && yl && yse && zl && zse);
+  return o->axpy_cplx (n, al, ase, xl, xse, yl, yse, zl, zse, (hipStream_t) stream);
+}
+
+extern "C" int mpgfx_gemv_real (int pb, int64_t m, int64_t k, const uint64_t *Al, const int64_t *Ase,
+                                const uint64_t *xl, const int64_t *xse, uint64_t *yl, int64_t *yse, void *stream)
+{
+  if (m < 0 || k < 0) return MPGFX_EINVAL;
+  GET_OPS (o);
+  if (m == 0) return MPGFX_OK;
+  NEED (yl && yse);
+  if (k > 0) NEED (Al && Ase && xl && xse);
+  return o->gemv (m, k, Al, Ase, xl, xse, yl, yse, (hipStream_t) stream);
+}
+
+extern "C" int mpgfx_gemm_workspace_size (int pb, int mode, int64_t m, int64_t n, int64_t k, size_t *bytes)
+{
+  if (!bytes || m < 0 || n < 0 || k < 0) return MPGFX_EINVAL;
+  if (mode != MPGFX_GEMM_PEROP && mode != MPGFX_GEMM_EXACT) return MPGFX_EINVAL;
+  if (!find_ops (pb)) return MPGFX_EUNSUPPORTED;
+  *bytes = 0;
+  if (mode == MPGFX_GEMM_EXACT) return MPGFX_EUNSUPPORTED;
+  return MPGFX_OK;
+}
+
+extern "C" int mpgfx_gemm_real (int pb, int mode, int64_t m, int64_t n, int64_t k, const uint64_t *Al,
+                                const int64_t *Ase, const uint64_t *Bl, const int64_t *Bse, uint64_t *Cl,
+                                int64_t *Cse, void *ws, size_t ws_bytes, void *stream)
+{
+  (void) ws; (void) ws_bytes;
+  if (m < 0 || n < 0 || k < 0) return MPGFX_EINVAL;
+  if (mode != MPGFX_GEMM_PEROP && mode != MPGFX_GEMM_EXACT) return MPGFX_EINVAL;
+  GET_OPS (o);
+  if (m == 0 || n == 0) return MPGFX_OK;
+  NEED (Cl && Cse);
+  if (k > 0) NEED (Al && Ase && Bl && Bse);
+  if (mode == MPGFX_GEMM_EXACT) return MPGFX_EUNSUPPORTED;
+  return o->gemm_perop (m, n, k, Al, Ase, Bl, Bse, Cl, Cse, (hipStream_t) stream);
+}
+
+extern "C" int mpgfx_gemm_complex (int pb, int64_t m, int64_t n, int64_t k, const uint64_t *Al, const int64_t *Ase,
+                                   const uint64_t *Bl, const int64_t *Bse, uint64_t *Cl, int64_t *Cse, void *stream)
+{
+  if (m < 0 || n < 0 || k < 0) return MPGFX_EINVAL;
+  GET_OPS (o);
+  if (m == 0 || n == 0) return MPGFX_OK;
+  NEED (Cl && Cse);
+  if (k > 0) NEED (Al && Ase && Bl && Bse);
+  return o->gemm_cplx (m, n, k, Al, Ase, Bl, Bse, Cl, Cse, (hipStream_t) stream);
+}
+
+extern "C" int mpgfx_pack (int pb, int cplx, int64_t n, const void *aos, uint64_t *l, int64_t *se, void *stream)
+{
+  if (n < 0) return MPGFX_EINVAL;
+  GET_OPS (o);
+  if (n == 0) return MPGFX_OK;
+  NEED (aos && l && se);
+  return o->pack (cplx ? 1 : 0, n, aos, l, se, (hipStream_t) stream);
+}
+
+extern "C" int mpgfx_unpack (int pb, int cplx, int64_t n, const uint64_t *l, const int64_t *se, void *aos,
+                             void *stream)
+{
+  if (n < 0) return MPGFX_EINVAL;
+  GET_OPS (o);
+  if (n == 0) return MPGFX_OK;
+  NEED (aos && l && se);
+  return o->unpack (cplx ? 1 : 0, n, l, se, aos, (hipStream_t) stream);
+}

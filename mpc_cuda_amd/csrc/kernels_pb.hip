@@ -1,0 +1,291 @@
+/* kernels_pb.hip -- the per-precision kernels of the mpgfx library, compiled
+ * once per precision with -DMPGFX_PB=<bits> (one translation unit per PB, so
+ * the precisions build in parallel).
+ *
+ *  elementwise real / complex   cu_fmul, cu_fadd, cu_fsub / cu_cmul, cu_cadd,
+ *                               cu_csub per element  (tools/test_freal.cu:41-48,
+ *                               tools/test_fcomplex.cu:147-153)
+ *  AXPY real / complex          demos/bench_3way.cu:44-59
+ *  GEMM per-op, real / complex  tools/linalg_fast.cu:53-68 (k-ordered fold)
+ *  GEMV per-op                  tools/linalg_fast.cu:37-51
+ *  AoS <-> SoA                  reference struct layout cu_freal.cuh:75-77
+ *
+ * Mapping: one value per lane.  Each lane keeps its significands as 32-bit
+ * words in VGPRs and runs the cu_fp word engine; scalars broadcast to the whole
+ * grid (the AXPY coefficient) are read at wave-uniform addresses so they live
+ * in SGPRs.  Streamed arrays use non-temporal loads/stores.
+ */
+#include "common.h"
+#include "dispatch.h"
+
+#ifndef MPGFX_PB
+#error "compile with -DMPGFX_PB=<precision in bits>"
+#endif
+
+namespace mpgfx {
+namespace {
+
+constexpr int PB = MPGFX_PB;
+constexpr int NL = cu_freal<PB>::N;
+/* workgroup size: 256 lanes (one wave per SIMD of a CU) lets the compiler use
+ * up to 512 registers per lane for the widest precisions */
+constexpr int TPB = 256;
+
+template <int OP>
+__global__ __launch_bounds__ (TPB) void k_ew_real (i64 n, const u64 *__restrict__ al, const i64 *__restrict__ ase,
+                                                   const u64 *__restrict__ bl, const i64 *__restrict__ bse,
+                                                   u64 *__restrict__ ol, i64 *__restrict__ ose)
+{
+  i64 stride = (i64) gridDim.x * blockDim.x;
+  for (i64 i = (i64) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+      cu_freal<PB> a = ld<PB> (al, ase, n, i), b = ld<PB> (bl, bse, n, i), r;
+      if (OP == MPGFX_MUL) r = cu_fp::cu_fmul<PB> (a, b);
+      else if (OP == MPGFX_ADD) r = cu_fp::cu_fadd<PB> (a, b);
+      else r = cu_fp::cu_fsub<PB> (a, b);
+      st<PB> (r, ol, ose, n, i);
+    }
+}
+
+template <int OP>
+__global__ __launch_bounds__ (TPB) void k_ew_cplx (i64 n, const u64 *__restrict__ al, const i64 *__restrict__ ase,
+                                                   const u64 *__restrict__ bl, const i64 *__restrict__ bse,
+                                                   u64 *__restrict__ ol, i64 *__restrict__ ose)
+{
+  const i64 o = (i64) NL * n;
+  i64 stride = (i64) gridDim.x * blockDim.x;
+  for (i64 i = (i64) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+      cu_fcomplex<PB> a (ld<PB> (al, ase, n, i), ld<PB> (al + o, ase + n, n, i));
+      cu_fcomplex<PB> b (ld<PB> (bl, bse, n, i), ld<PB> (bl + o, bse + n, n, i));
+      cu_fcomplex<PB> r;
+      if (OP == MPGFX_MUL) r = cu_fp::cu_cmul<PB> (a, b);
+      else if (OP == MPGFX_ADD) r = cu_fp::cu_cadd<PB> (a, b);
+      else r = cu_fp::cu_csub<PB> (a, b);
+      st<PB> (r.re, ol, ose, n, i);
+      st<PB> (r.im, ol + o, ose + n, n, i);
+    }
+}
+
+/* out = RNDN(RNDN(a*x) + y) */
+__global__ __launch_bounds__ (TPB) void k_axpy_real (i64 n, const u64 *__restrict__ al, const i64 *__restrict__ ase,
+                                                     const u64 *__restrict__ xl, const i64 *__restrict__ xse,
+                                                     const u64 *__restrict__ yl, const i64 *__restrict__ yse,
+                                                     u64 *__restrict__ ol, i64 *__restrict__ ose)
+{
+  const cu_freal<PB> a = ldc<PB> (al, ase, 1, 0);
+  i64 stride = (i64) gridDim.x * blockDim.x;
+  for (i64 i = (i64) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+      cu_freal<PB> t = cu_fp::cu_fmul<PB> (a, ld<PB> (xl, xse, n, i));
+      st<PB> (cu_fp::cu_fadd<PB> (t, ld<PB> (yl, yse, n, i)), ol, ose, n, i);
+    }
+}
+
+/* z = a*x + y, MPC_RNDNN per operation */
+__global__ __launch_bounds__ (TPB) void k_axpy_cplx (i64 n, const u64 *__restrict__ al, const i64 *__restrict__ ase,
+                                                     const u64 *__restrict__ xl, const i64 *__restrict__ xse,
+                                                     const u64 *__restrict__ yl, const i64 *__restrict__ yse,
+                                                     u64 *__restrict__ zl, i64 *__restrict__ zse)
+{
+  const i64 o = (i64) NL * n;
+  const cu_fcomplex<PB> a (ldc<PB> (al, ase, 1, 0), ldc<PB> (al + NL, ase + 1, 1, 0));
+  i64 stride = (i64) gridDim.x * blockDim.x;
+  for (i64 i = (i64) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+      cu_fcomplex<PB> x (ld<PB> (xl, xse, n, i), ld<PB> (xl + o, xse + n, n, i));
+      cu_fcomplex<PB> t = cu_fp::cu_cmul<PB> (a, x);
+      cu_fcomplex<PB> y (ld<PB> (yl, yse, n, i), ld<PB> (yl + o, yse + n, n, i));
+      cu_fcomplex<PB> r = cu_fp::cu_cadd<PB> (t, y);
+      st<PB> (r.re, zl, zse, n, i);
+      st<PB> (r.im, zl + o, zse + n, n, i);
+    }
+}
+
+/* C[i,j] = fold_k RNDN(acc + RNDN(A[i,k]*B[k,j])): one lane per C element; the
+ * lanes of a workgroup share the row i, so A[i,k] is one broadcast load and
+ * B[k, j..j+255] one coalesced row segment per step. */
+__global__ __launch_bounds__ (TPB) void k_gemm_perop (i64 m, i64 n, i64 k, const u64 *__restrict__ Al,
+                                                      const i64 *__restrict__ Ase, const u64 *__restrict__ Bl,
+                                                      const i64 *__restrict__ Bse, u64 *__restrict__ Cl,
+                                                      i64 *__restrict__ Cse)
+{
+  const i64 tiles = (n + TPB - 1) / TPB;
+  const i64 nA = m * k, nB = k * n, nC = m * n;
+  for (i64 t = blockIdx.x; t < m * tiles; t += gridDim.x)
+    {
+      const i64 i = t / tiles, j = (t % tiles) * TPB + threadIdx.x;
+      const bool on = j < n;
+      const i64 jj = on ? j : 0;
+      cu_freal<PB> acc;
+      acc.set_zero ();
+      for (i64 kk = 0; kk < k; kk++)
+        {
+          cu_freal<PB> a = ldc<PB> (Al, Ase, nA, i * k + kk);
+          cu_freal<PB> b = ldc<PB> (Bl, Bse, nB, kk * n + jj);
+          acc = cu_fp::cu_fadd<PB> (acc, cu_fp::cu_fmul<PB> (a, b));
+        }
+      if (on) st<PB> (acc, Cl, Cse, nC, i * n + j);
+    }
+}
+
+__global__ __launch_bounds__ (TPB) void k_gemm_cplx (i64 m, i64 n, i64 k, const u64 *__restrict__ Al,
+                                                     const i64 *__restrict__ Ase, const u64 *__restrict__ Bl,
+                                                     const i64 *__restrict__ Bse, u64 *__restrict__ Cl,
+                                                     i64 *__restrict__ Cse)
+{
+  const i64 tiles = (n + TPB - 1) / TPB;
+  const i64 nA = m * k, nB = k * n, nC = m * n;
+  const i64 oA = (i64) NL * nA, oB = (i64) NL * nB, oC = (i64) NL * nC;
+  for (i64 t = blockIdx.x; t < m * tiles; t += gridDim.x)
+    {
+      const i64 i = t / tiles, j = (t % tiles) * TPB + threadIdx.x;
+      const bool on = j < n;
+      const i64 jj = on ? j : 0;
+      cu_fcomplex<PB> acc;
+      acc.re.set_zero ();
+      acc.im.set_zero ();
+      for (i64 kk = 0; kk < k; kk++)
+        {
+          i64 ia = i * k + kk, ib = kk * n + jj;
+          cu_fcomplex<PB> a (ldc<PB> (Al, Ase, nA, ia), ldc<PB> (Al + oA, Ase + nA, nA, ia));
+          cu_fcomplex<PB> b (ldc<PB> (Bl, Bse, nB, ib), ldc<PB> (Bl + oB, Bse + nB, nB, ib));
+          acc = cu_fp::cu_cadd<PB> (acc, cu_fp::cu_cmul<PB> (a, b));
+        }
+      if (on)
+        {
+          st<PB> (acc.re, Cl, Cse, nC, i * n + j);
+          st<PB> (acc.im, Cl + oC, Cse + nC, nC, i * n + j);
+        }
+    }
+}
+
+/* y[i] = fold_k RNDN(acc + RNDN(A[i,k]*x[k])): one lane per row; x[k] is a
+ * broadcast load shared by the whole grid. */
+__global__ __launch_bounds__ (TPB) void k_gemv (i64 m, i64 k, const u64 *__restrict__ Al, const i64 *__restrict__ Ase,
+                                                const u64 *__restrict__ xl, const i64 *__restrict__ xse,
+                                                u64 *__restrict__ yl, i64 *__restrict__ yse)
+{
+  const i64 nA = m * k;
+  for (i64 i = (i64) blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (i64) gridDim.x * blockDim.x)
+    {
+      cu_freal<PB> acc;
+      acc.set_zero ();
+      for (i64 kk = 0; kk < k; kk++)
+        {
+          cu_freal<PB> a = ldc<PB> (Al, Ase, nA, i * k + kk);
+          cu_freal<PB> x = ldc<PB> (xl, xse, k, kk);
+          acc = cu_fp::cu_fadd<PB> (acc, cu_fp::cu_fmul<PB> (a, x));
+        }
+      st<PB> (acc, yl, yse, m, i);
+    }
+}
+
+/* AoS (reference struct layout) <-> SoA, one lane per real value */
+__global__ void k_pack (i64 n, int cplx, const cu_freal<PB> *__restrict__ aos, u64 *__restrict__ l,
+                        i64 *__restrict__ se)
+{
+  i64 i = (i64) blockIdx.x * blockDim.x + threadIdx.x;
+  i64 tot = cplx ? 2 * n : n;
+  if (i >= tot) return;
+  i64 part = (cplx && i >= n) ? 1 : 0, j = i - part * n;
+  i64 src = cplx ? 2 * j + part : i;
+  cu_freal<PB> v = aos[src];
+  st<PB> (v, l + part * (i64) NL * n, se + part * n, n, j);
+}
+__global__ void k_unpack (i64 n, int cplx, const u64 *__restrict__ l, const i64 *__restrict__ se,
+                          cu_freal<PB> *__restrict__ aos)
+{
+  i64 i = (i64) blockIdx.x * blockDim.x + threadIdx.x;
+  i64 tot = cplx ? 2 * n : n;
+  if (i >= tot) return;
+  i64 part = (cplx && i >= n) ? 1 : 0, j = i - part * n;
+  i64 dst = cplx ? 2 * j + part : i;
+  aos[dst] = ldc<PB> (l + part * (i64) NL * n, se + part * n, n, j);
+}
+
+unsigned grid_for (i64 work, int threads, int per_cu)
+{
+  i64 b = (work + threads - 1) / threads, cap = (i64) 256 * per_cu;
+  return (unsigned) (b > cap ? cap : (b < 1 ? 1 : b));
+}
+
+int ew_real (int op, i64 n, const u64 *al, const i64 *ase, const u64 *bl, const i64 *bse, u64 *ol, i64 *ose,
+             hipStream_t s)
+{
+  unsigned g = grid_for (n, TPB, 32);
+  if (op == MPGFX_MUL) hipLaunchKernelGGL ((k_ew_real<MPGFX_MUL>), dim3 (g), dim3 (TPB), 0, s, n, al, ase, bl, bse, ol, ose);
+  else if (op == MPGFX_ADD) hipLaunchKernelGGL ((k_ew_real<MPGFX_ADD>), dim3 (g), dim3 (TPB), 0, s, n, al, ase, bl, bse, ol, ose);
+  else hipLaunchKernelGGL ((k_ew_real<MPGFX_SUB>), dim3 (g), dim3 (TPB), 0, s, n, al, ase, bl, bse, ol, ose);
+  return launch_status ("elementwise_real");
+}
+int ew_cplx (int op, i64 n, const u64 *al, const i64 *ase, const u64 *bl, const i64 *bse, u64 *ol, i64 *ose,
+             hipStream_t s)
+{
+  unsigned g = grid_for (n, TPB, 32);
+  if (op == MPGFX_MUL) hipLaunchKernelGGL ((k_ew_cplx<MPGFX_MUL>), dim3 (g), dim3 (TPB), 0, s, n, al, ase, bl, bse, ol, ose);
+  else if (op == MPGFX_ADD) hipLaunchKernelGGL ((k_ew_cplx<MPGFX_ADD>), dim3 (g), dim3 (TPB), 0, s, n, al, ase, bl, bse, ol, ose);
+  else hipLaunchKernelGGL ((k_ew_cplx<MPGFX_SUB>), dim3 (g), dim3 (TPB), 0, s, n, al, ase, bl, bse, ol, ose);
+  return launch_status ("elementwise_complex");
+}
+int axpy_real (i64 n, const u64 *al, const i64 *ase, const u64 *xl, const i64 *xse, const u64 *yl, const i64 *yse,
+               u64 *ol, i64 *ose, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_axpy_real, dim3 (grid_for (n, TPB, 32)), dim3 (TPB), 0, s, n, al, ase, xl, xse, yl, yse, ol, ose);
+  return launch_status ("axpy_real");
+}
+int axpy_cplx (i64 n, const u64 *al, const i64 *ase, const u64 *xl, const i64 *xse, const u64 *yl, const i64 *yse,
+               u64 *ol, i64 *ose, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_axpy_cplx, dim3 (grid_for (n, TPB, 32)), dim3 (TPB), 0, s, n, al, ase, xl, xse, yl, yse, ol, ose);
+  return launch_status ("axpy_complex");
+}
+int gemm_perop (i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl, const i64 *Bse, u64 *Cl,
+                i64 *Cse, hipStream_t s)
+{
+  i64 tiles = m * ((n + TPB - 1) / TPB);
+  unsigned g = (unsigned) (tiles > 65536 ? 65536 : tiles);
+  hipLaunchKernelGGL (k_gemm_perop, dim3 (g), dim3 (TPB), 0, s, m, n, k, Al, Ase, Bl, Bse, Cl, Cse);
+  return launch_status ("gemm_real");
+}
+int gemm_cplx (i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl, const i64 *Bse, u64 *Cl,
+               i64 *Cse, hipStream_t s)
+{
+  i64 tiles = m * ((n + TPB - 1) / TPB);
+  unsigned g = (unsigned) (tiles > 65536 ? 65536 : tiles);
+  hipLaunchKernelGGL (k_gemm_cplx, dim3 (g), dim3 (TPB), 0, s, m, n, k, Al, Ase, Bl, Bse, Cl, Cse);
+  return launch_status ("gemm_complex");
+}
+int gemv (i64 m, i64 k, const u64 *Al, const i64 *Ase, const u64 *xl, const i64 *xse, u64 *yl, i64 *yse,
+          hipStream_t s)
+{
+  /* 64-lane workgroups: one row per lane, as many workgroups as rows allow */
+  unsigned g = (unsigned) ((m + 63) / 64);
+  hipLaunchKernelGGL (k_gemv, dim3 (g), dim3 (64), 0, s, m, k, Al, Ase, xl, xse, yl, yse);
+  return launch_status ("gemv_real");
+}
+int pack (int cplx, i64 n, const void *aos, u64 *l, i64 *se, hipStream_t s)
+{
+  i64 tot = cplx ? 2 * n : n;
+  hipLaunchKernelGGL (k_pack, dim3 ((unsigned) ((tot + 255) / 256)), dim3 (256), 0, s, n, cplx,
+                      (const cu_freal<PB> *) aos, l, se);
+  return launch_status ("pack");
+}
+int unpack (int cplx, i64 n, const u64 *l, const i64 *se, void *aos, hipStream_t s)
+{
+  i64 tot = cplx ? 2 * n : n;
+  hipLaunchKernelGGL (k_unpack, dim3 ((unsigned) ((tot + 255) / 256)), dim3 (256), 0, s, n, cplx, l, se,
+                      (cu_freal<PB> *) aos);
+  return launch_status ("unpack");
+}
+
+const pb_ops ops = { PB, ew_real, ew_cplx, axpy_real, axpy_cplx, gemm_perop, gemm_cplx, gemv, pack, unpack };
+
+} // namespace
+
+/* registration hook, one per compiled precision */
+#define MPGFX_CAT2(a, b) a##b
+#define MPGFX_CAT(a, b) MPGFX_CAT2 (a, b)
+const pb_ops *MPGFX_CAT (ops_pb_, MPGFX_PB) () { return &ops; }
+
+} // namespace mpgfx

@@ -1,0 +1,108 @@
+"""Host-side calls into the HIP library on device MP arrays.
+
+These mirror the reference's operations on cu_fp::cu_freal<PB> /
+cu_fcomplex<PB> (tkouya/mpc_cuda include/mpc_cuda/cu_freal.cuh,
+cu_fcomplex.cuh, demos/bench_3way.cu, tools/linalg_fast.cu), each launched
+asynchronously on the current torch stream.  Arguments must be device
+MPArrays of matching precision; results are new device MPArrays.
+"""
+import ctypes
+
+from . import _lib
+from .arrays import MPArray, empty
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev(x):
+    if not isinstance(x, MPArray) or not x.on_device:
+        raise TypeError("expected a device MPArray (use .cuda())")
+    return x
+
+
+def _same(pb, *xs):
+    for x in xs:
+        if x.pb != pb:
+            raise ValueError("precision mismatch")
+
+
+_OPS = {"mul": _lib.MUL, "add": _lib.ADD, "sub": _lib.SUB}
+
+
+def elementwise(op, a, b, out=None):
+    """out = a OP b elementwise (real: cu_fmul/cu_fadd/cu_fsub; complex:
+    cu_cmul/cu_cadd/cu_csub)."""
+    _dev(a), _dev(b)
+    _same(a.pb, b)
+    if a.cplx != b.cplx or a.n != b.n:
+        raise ValueError("shape mismatch")
+    out = out if out is not None else empty(a.pb, a.n, a.cplx, device=a.se.device)
+    L = _lib.lib()
+    f = L.mpgfx_elementwise_complex if a.cplx else L.mpgfx_elementwise_real
+    _lib.check(f.__name__, f(a.pb, _OPS[op], a.n, *a.ptrs(), *b.ptrs(), *out.ptrs(), _stream()))
+    return out
+
+
+def axpy(a, x, y, out=None):
+    """out = RNDN(RNDN(a*x) + y) with a scalar (an MPArray of one value)."""
+    _dev(a), _dev(x), _dev(y)
+    _same(x.pb, a, y)
+    if a.n != 1 or x.n != y.n or not (a.cplx == x.cplx == y.cplx):
+        raise ValueError("shape mismatch")
+    out = out if out is not None else empty(x.pb, x.n, x.cplx, device=x.se.device)
+    L = _lib.lib()
+    f = L.mpgfx_axpy_complex if x.cplx else L.mpgfx_axpy_real
+    _lib.check(f.__name__, f(x.pb, x.n, *a.ptrs(), *x.ptrs(), *y.ptrs(), *out.ptrs(), _stream()))
+    return out
+
+
+def gemv(A, x, m, k, out=None):
+    """y = A x with per-operation rounding in k order (A: m*k values)."""
+    _dev(A), _dev(x)
+    _same(A.pb, x)
+    if A.n != m * k or x.n != k or A.cplx or x.cplx:
+        raise ValueError("shape mismatch")
+    out = out if out is not None else empty(A.pb, m, False, device=A.se.device)
+    L = _lib.lib()
+    _lib.check("mpgfx_gemv_real", L.mpgfx_gemv_real(A.pb, m, k, *A.ptrs(), *x.ptrs(), *out.ptrs(), _stream()))
+    return out
+
+
+def gemm(A, B, m, n, k, mode="perop", out=None, workspace=None):
+    """C = A B (A: m*k, B: k*n values, row-major).  mode 'perop' (the
+    reference's matmul order) or 'exact' (exact sum, one rounding); complex
+    arrays use the per-op complex GEMM."""
+    _dev(A), _dev(B)
+    _same(A.pb, B)
+    if A.n != m * k or B.n != k * n or A.cplx != B.cplx:
+        raise ValueError("shape mismatch")
+    out = out if out is not None else empty(A.pb, m * n, A.cplx, device=A.se.device)
+    L = _lib.lib()
+    if A.cplx:
+        if mode != "perop":
+            raise ValueError("complex GEMM supports mode='perop'")
+        _lib.check("mpgfx_gemm_complex",
+                   L.mpgfx_gemm_complex(A.pb, m, n, k, *A.ptrs(), *B.ptrs(), *out.ptrs(), _stream()))
+        return out
+    md = {"perop": _lib.GEMM_PEROP, "exact": _lib.GEMM_EXACT}[mode]
+    ws_ptr, ws_bytes = None, 0
+    if workspace is not None:
+        ws_ptr, ws_bytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+    _lib.check("mpgfx_gemm_real",
+               L.mpgfx_gemm_real(A.pb, md, m, n, k, *A.ptrs(), *B.ptrs(), *out.ptrs(),
+                                 ws_ptr, ws_bytes, _stream()))
+    return out
+
+
+def gemm_workspace_size(pb, mode, m, n, k):
+    md = {"perop": _lib.GEMM_PEROP, "exact": _lib.GEMM_EXACT}[mode]
+    b = ctypes.c_size_t(0)
+    _lib.check("mpgfx_gemm_workspace_size", _lib.lib().mpgfx_gemm_workspace_size(pb, md, m, n, k, ctypes.byref(b)))
+    return b.value
+
+
+def supported_pb(pb):
+    return bool(_lib.lib().mpgfx_supported_pb(pb))

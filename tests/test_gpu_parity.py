@@ -1,0 +1,151 @@
+"""Parity of the HIP library (through the C ABI) with the oracle and with the
+MPFR/MPC golden fixtures.  Bit-exact: every limb and the sign/exponent word,
+including the sign field of zeros, against the oracle (a restatement of the
+reference); zeros compared by value against MPFR fixtures (the reference's own
+tests do the same: tools/test_freal.cu:73)."""
+import numpy as np
+import pytest
+
+import mpc_cuda_amd as M
+from mpc_cuda_amd.arrays import MPArray
+from oracle import oracle as O
+import goldens
+from mpcases import edge_arrays
+
+pytestmark = pytest.mark.gpu
+PBS = goldens.PBS
+
+
+def dev(pair, pb, cplx=False):
+    return MPArray(pb, np.ascontiguousarray(pair[0]), np.ascontiguousarray(pair[1]), cplx).cuda()
+
+
+def host_pair(x):
+    return x.pair()
+
+
+def same(gpu_arr, oracle_pair):
+    return O.mismatches(host_pair(gpu_arr), oracle_pair, ignore_zero_sign=False)
+
+
+def same_value(gpu_arr, golden_pair):
+    return O.mismatches(host_pair(gpu_arr), golden_pair, ignore_zero_sign=True)
+
+
+@pytest.mark.parametrize("pb", PBS)
+def test_elementwise_real_vs_golden_and_oracle(gpu, pb):
+    g = goldens.load(pb)
+    a, b = goldens.pair(g, "a"), goldens.pair(g, "b")
+    da, db = dev(a, pb), dev(b, pb)
+    for op in ("mul", "add", "sub"):
+        r = M.elementwise(op, da, db)
+        assert same_value(r, goldens.pair(g, op)) == 0, op
+        assert same(r, O.elementwise_real(pb, op, a, b)) == 0, op
+    # the reference's generator and spread mix, plus edge cases
+    n = 30000 if pb <= 1056 else 8000
+    l, s = O.fill_ref(pb, 2 * n, 0x1234 + pb, -1)
+    ea, eb, _, _ = edge_arrays(pb, seed=13)
+    a = (np.concatenate([l[:, 0::2], ea.limbs], 1), np.concatenate([s[0::2], ea.se]))
+    b = (np.concatenate([l[:, 1::2], eb.limbs], 1), np.concatenate([s[1::2], eb.se]))
+    a = tuple(np.ascontiguousarray(v) for v in a)
+    b = tuple(np.ascontiguousarray(v) for v in b)
+    da, db = dev(a, pb), dev(b, pb)
+    for op in ("mul", "add", "sub"):
+        assert same(M.elementwise(op, da, db), O.elementwise_real(pb, op, a, b)) == 0, op
+
+
+@pytest.mark.parametrize("pb", PBS)
+def test_elementwise_complex_vs_golden_and_oracle(gpu, pb):
+    g = goldens.load(pb)
+    ca, cb = goldens.pair(g, "ca"), goldens.pair(g, "cb")
+    da, db = dev(ca, pb, True), dev(cb, pb, True)
+    for op in ("mul", "add", "sub"):
+        r = M.elementwise(op, da, db)
+        assert same_value(r, goldens.pair(g, "c" + op)) == 0, op
+        assert same(r, O.elementwise_complex(pb, op, ca, cb)) == 0, op
+    n = 6000 if pb <= 1056 else 2000
+    x = M.random_complex(pb, n, 77 + pb, -40, 40)
+    y = M.random_complex(pb, n, 78 + pb, -40, 40)
+    dx, dy = x.cuda(), y.cuda()
+    for op in ("mul", "add", "sub"):
+        assert same(M.elementwise(op, dx, dy), O.elementwise_complex(pb, op, x.pair(), y.pair())) == 0, op
+
+
+@pytest.mark.parametrize("pb", PBS)
+def test_axpy_vs_golden_and_oracle(gpu, pb):
+    g = goldens.load(pb)
+    r = M.axpy(dev(goldens.pair(g, "ax"), pb), dev(goldens.pair(g, "xx"), pb), dev(goldens.pair(g, "yy"), pb))
+    assert same_value(r, goldens.pair(g, "axpy")) == 0
+    r = M.axpy(dev(goldens.pair(g, "cax"), pb, True), dev(goldens.pair(g, "cxx"), pb, True),
+               dev(goldens.pair(g, "cyy"), pb, True))
+    assert same_value(r, goldens.pair(g, "caxpy")) == 0
+    # ragged length (not a multiple of the workgroup), oracle bit-exact
+    n = 4099
+    a = M.random_real(pb, 1, 5, -8, 8)
+    x = M.random_real(pb, n, 6, -8, 8)
+    y = M.random_real(pb, n, 7, -8, 8)
+    assert same(M.axpy(a.cuda(), x.cuda(), y.cuda()), O.axpy_real(pb, a.pair(), x.pair(), y.pair())) == 0
+    ca = M.random_complex(pb, 1, 8, -16, 16)
+    cx = M.random_complex(pb, 1031, 9, -16, 16)
+    cy = M.random_complex(pb, 1031, 10, -16, 16)
+    assert same(M.axpy(ca.cuda(), cx.cuda(), cy.cuda()), O.axpy_complex(pb, ca.pair(), cx.pair(), cy.pair())) == 0
+
+
+@pytest.mark.parametrize("pb", PBS)
+def test_gemm_gemv_perop_vs_golden_and_oracle(gpu, pb):
+    g = goldens.load(pb)
+    m, n, k = (int(v) for v in g["dims"])
+    for tag in ("g4", "g80"):
+        A, B = goldens.pair(g, tag + "A"), goldens.pair(g, tag + "B")
+        r = M.gemm(dev(A, pb), dev(B, pb), m, n, k, "perop")
+        assert same_value(r, goldens.pair(g, tag + "perop")) == 0, tag
+        assert same(r, O.gemm(pb, m, n, k, A, B, "perop")) == 0, tag
+    r = M.gemm(dev(goldens.pair(g, "cgA"), pb, True), dev(goldens.pair(g, "cgB"), pb, True), m, n, k)
+    assert same_value(r, goldens.pair(g, "cgperop")) == 0
+    # GEMV: y = A x, with a ragged row count
+    mm, kk = 67, 45
+    A = M.random_real(pb, mm * kk, 3 + pb, -4, 4)
+    x = M.random_real(pb, kk, 4 + pb, -4, 4)
+    r = M.gemv(A.cuda(), x.cuda(), mm, kk)
+    assert same(r, O.gemm(pb, mm, 1, kk, A.pair(), x.pair(), "perop")) == 0
+
+
+def test_empty_and_single_element(gpu):
+    pb = 256
+    a = M.random_real(pb, 1, 1).cuda()
+    e = M.empty(pb, 0)
+    r = M.axpy(a, e, e)
+    assert r.n == 0
+    x = M.random_real(pb, 1, 2)
+    y = M.random_real(pb, 1, 3)
+    r = M.axpy(a, x.cuda(), y.cuda())
+    assert same(r, O.axpy_real(pb, a.pair(), x.pair(), y.pair())) == 0
+    # k = 0 GEMM gives +0 everywhere (acc starts at +0)
+    A = M.empty(pb, 0)
+    C = M.gemm(A, A, 3, 2, 0)
+    h = C.numpy()
+    assert np.all(h.limbs == 0) and np.all(h.se == M.ZERO_EXP * 2)
+
+
+@pytest.mark.parametrize("pb,cplx", [(256, False), (1024, False), (96, True), (512, True)])
+def test_pack_unpack_roundtrip(gpu, pb, cplx):
+    import ctypes
+    import torch
+    n = 1000
+    x = M.random_array(pb, n, 5, -30, 30, cplx).cuda()
+    N = M.nlimbs(pb)
+    aos = torch.zeros(((2 if cplx else 1) * n * (2 + N),), dtype=torch.int64, device="cuda")
+    L = M.lib()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert L.mpgfx_unpack(pb, int(cplx), n, *x.ptrs(), aos.data_ptr(), s) == 0
+    # AoS struct: {int sign; <pad>; long exp; cu_limb m[N]}
+    h = aos.cpu().numpy().reshape(-1, 2 + N)
+    xr = x.numpy()
+    re_se = xr.se[0] if cplx else xr.se
+    first = h[0::2] if cplx else h
+    assert np.all(first[:, 1] == (re_se >> 1))
+    sign = first[:, 0].astype(np.int64) & 0xFFFFFFFF
+    assert np.all(np.where(re_se & 1, sign == 0xFFFFFFFF, sign == 1))
+    y = M.empty(pb, n, cplx)
+    assert L.mpgfx_pack(pb, int(cplx), n, aos.data_ptr(), *y.ptrs(), s) == 0
+    assert M.equal(x, y) == 0
