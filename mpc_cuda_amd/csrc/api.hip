@@ -102,8 +102,7 @@ extern "C" int mpgfx_gemm_workspace_size (int pb, int mode, int64_t m, int64_t n
   if (!bytes || m < 0 || n < 0 || k < 0) return MPGFX_EINVAL;
   if (mode != MPGFX_GEMM_PEROP && mode != MPGFX_GEMM_EXACT) return MPGFX_EINVAL;
   if (!find_ops (pb)) return MPGFX_EUNSUPPORTED;
-  *bytes = 0;
-  if (mode == MPGFX_GEMM_EXACT) return MPGFX_EUNSUPPORTED;
+  *bytes = (mode == MPGFX_GEMM_EXACT) ? exact_workspace_bound (pb, m, n, k) : 0;
   return MPGFX_OK;
 }
 
@@ -111,14 +110,14 @@ extern "C" int mpgfx_gemm_real (int pb, int mode, int64_t m, int64_t n, int64_t 
                                 const int64_t *Ase, const uint64_t *Bl, const int64_t *Bse, uint64_t *Cl,
                                 int64_t *Cse, void *ws, size_t ws_bytes, void *stream)
 {
-  (void) ws; (void) ws_bytes;
   if (m < 0 || n < 0 || k < 0) return MPGFX_EINVAL;
   if (mode != MPGFX_GEMM_PEROP && mode != MPGFX_GEMM_EXACT) return MPGFX_EINVAL;
   GET_OPS (o);
   if (m == 0 || n == 0) return MPGFX_OK;
   NEED (Cl && Cse);
   if (k > 0) NEED (Al && Ase && Bl && Bse);
-  if (mode == MPGFX_GEMM_EXACT) return MPGFX_EUNSUPPORTED;
+  if (mode == MPGFX_GEMM_EXACT)
+    return gemm_exact (o, pb, m, n, k, Al, Ase, Bl, Bse, Cl, Cse, ws, ws_bytes, (hipStream_t) stream);
   return o->gemm_perop (m, n, k, Al, Ase, Bl, Bse, Cl, Cse, (hipStream_t) stream);
 }
 

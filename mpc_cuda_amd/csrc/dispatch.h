@@ -3,6 +3,7 @@
 #define MPGFX_DISPATCH_H
 
 #include "common.h"
+#include "exact.h"
 
 namespace mpgfx {
 
@@ -22,6 +23,14 @@ struct pb_ops
   int (*gemv) (i64 m, i64 k, const u64 *, const i64 *, const u64 *, const i64 *, u64 *, i64 *, hipStream_t);
   int (*pack) (int cplx, i64 n, const void *aos, u64 *, i64 *, hipStream_t);
   int (*unpack) (int cplx, i64 n, const u64 *, const i64 *, void *aos, hipStream_t);
+  /* exact-dot GEMM stages (exact_pb.h) */
+  int (*exact_digits) (i64 lines, i64 len, i64 ls, i64 ps, const u64 *l, const i64 *se, i64 nv, const i64 *emax,
+                       i64 spread, int S, int Spad, i64 Lpad, i64 kp, int8_t *D, hipStream_t);
+  int (*exact_mfma) (int G, const exact_plan &p, const int8_t *DA, const int8_t *DB, const i64 *emaxA,
+                     const i64 *emaxB, i64 *slotbuf, int grid, u64 *Cl, i64 *Cse, hipStream_t);
+  int (*exact_fallback) (i64 m, i64 n, i64 k, i64 r0, i64 rows, const u64 *Al, const i64 *Ase, const u64 *Bl,
+                         const i64 *Bse, const i64 *eminA, const i64 *eminB, int nw, i64 *wsp, u64 *Cl, i64 *Cse,
+                         hipStream_t);
 };
 
 #define MPGFX_DECL(P) const pb_ops *ops_pb_##P ();
@@ -30,6 +39,11 @@ MPGFX_PB_LIST (MPGFX_DECL)
 
 /* table lookup; NULL if pb is not compiled in */
 const pb_ops *find_ops (int pb);
+
+/* exact-dot GEMM (exact_gemm.hip) */
+size_t exact_workspace_bound (int pb, i64 m, i64 n, i64 k);
+int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl,
+                const i64 *Bse, u64 *Cl, i64 *Cse, void *ws, size_t ws_bytes, hipStream_t s);
 
 } // namespace mpgfx
 #endif

@@ -1,0 +1,311 @@
+/* exact_pb.h -- per-precision kernels of the exact-dot GEMM (see exact.h for
+ * the scheme).  Included by kernels_pb.hip inside its anonymous namespace,
+ * where PB (bits) and NL (u64 limbs) are defined. */
+
+constexpr int KWD = PB / 32;                   /* significand words (pb bits) */
+constexpr int SBW = cu_freal<PB>::SB / 32;     /* zero low words of storage   */
+
+typedef int v4i __attribute__ ((ext_vector_type (4)));
+
+/* load the pb-bit significand M' (storage >> SB) as KWD words */
+__device__ __forceinline__ void load_sig (const u64 *__restrict__ l, i64 stride, i64 i, u32 (&w)[KWD])
+{
+  u32 all[2 * NL];
+#pragma unroll
+  for (int q = 0; q < NL; q++)
+    {
+      u64 v = l[(i64) q * stride + i];
+      all[2 * q] = (u32) v;
+      all[2 * q + 1] = (u32) (v >> 32);
+    }
+#pragma unroll
+  for (int q = 0; q < KWD; q++) w[q] = all[q + SBW];
+}
+
+/* Digit planes of one operand.  Line L (row of A / column of B), position p
+ * (k index); the value sits at index L*ls + p*ps of the MP array (nv values).
+ * D[t][L][p] for t < Spad, L < Lpad, p < kp; padding and planes >= S are 0. */
+__global__ __launch_bounds__ (256) void k_exact_digits (i64 lines, i64 len, i64 ls, i64 ps, const u64 *__restrict__ l,
+                                                        const i64 *__restrict__ se, i64 nv,
+                                                        const i64 *__restrict__ emax_line, i64 spread, int S,
+                                                        int Spad, i64 Lpad, i64 kp, int8_t *__restrict__ D)
+{
+  const i64 p = (i64) blockIdx.x * blockDim.x + threadIdx.x;
+  const i64 L = blockIdx.y;
+  if (p >= kp || L >= Lpad) return;
+  const i64 plane = Lpad * kp;
+  int8_t *out = D + L * kp + p;
+  bool live = (L < lines) && (p < len);
+  i64 idx = live ? L * ls + p * ps : 0;
+  i64 w = live ? se[idx] : 0;
+  i64 e = w >> 1;
+  i64 em = live ? emax_line[L] : EXACT_EMPTY;
+  live = live && (e != MPGFX_ZERO_EXP) && (em != EXACT_EMPTY);
+  if (!live)
+    {
+      for (int t = 0; t < Spad; t++) out[(i64) t * plane] = 0;
+      return;
+    }
+  u32 M[KWD];
+  load_sig (l, nv, idx, M);
+  const i64 sh = spread - (em - e);              /* 0 <= sh <= spread */
+  const int q = (int) (sh >> 3), r = (int) (sh & 7);
+  u32 X[KWD + 1];                                 /* M' << r */
+#pragma unroll
+  for (int j = KWD; j >= 0; j--)
+    {
+      u32 hi = (j < KWD) ? M[j] : 0u, lo = (j >= 1) ? M[j - 1] : 0u;
+      X[j] = cu_fp::core::fshl (hi, lo, (u32) r);
+    }
+  const u32 neg = (u32) (w & 1);
+  u32 cn = neg, c = 0;                            /* negation carry, recode carry */
+  for (int t = 0; t < q && t < Spad; t++) out[(i64) t * plane] = 0;
+#pragma unroll
+  for (int b = 0; b < 4 * (KWD + 1); b++)
+    {
+      const int t = q + b;
+      u32 y = (X[b >> 2] >> (8 * (b & 3))) & 255u;
+      if (neg) { y = (~y & 255u) + cn; cn = y >> 8; y &= 255u; }
+      int d;
+      if (t == S - 1)
+        { d = (int) (y >= 128u ? (int) y - 256 : (int) y) + (int) c; c = 0; }
+      else
+        { u32 v = y + c; c = v >= 128u; d = (int) v - (c ? 256 : 0); }
+      if (t < S) out[(i64) t * plane] = (int8_t) d;
+    }
+  for (int t = q + 4 * (KWD + 1); t < S; t++)
+    {
+      u32 y = 0;
+      if (neg) { y = 255u + cn; cn = y >> 8; y &= 255u; }
+      int d;
+      if (t == S - 1)
+        { d = (int) (y >= 128u ? (int) y - 256 : (int) y) + (int) c; c = 0; }
+      else
+        { u32 v = y + c; c = v >= 128u; d = (int) v - (c ? 256 : 0); }
+      out[(i64) t * plane] = (int8_t) d;
+    }
+  for (int t = (S > q ? S : q); t < Spad; t++) out[(i64) t * plane] = 0;
+}
+
+/* Round the two's complement integer held in the first `nw` 32-bit words of
+ * wbuf (stored one per 64-bit slot) times 2^scale to pb bits and store it as
+ * value i of C.  The words are read at runtime positions from memory, so no
+ * register array is indexed dynamically. */
+__device__ void round_store_int (i64 *__restrict__ wbuf, int nw, i64 scale, u64 *__restrict__ Cl,
+                                 i64 *__restrict__ Cse, i64 nC, i64 i)
+{
+  /* sign from the top word, then magnitude in place */
+  const bool neg = ((u32) wbuf[nw - 1] >> 31) != 0;
+  if (neg)
+    {
+      u32 c = 1;
+      for (int s = 0; s < nw; s++)
+        {
+          u32 v = ~(u32) wbuf[s];
+          wbuf[s] = (i64) cu_fp::core::addc (v, 0u, c);
+        }
+    }
+  int h = -1;
+  for (int s = nw - 1; s >= 0; s--)
+    if ((u32) wbuf[s] != 0u) { h = s; break; }
+  cu_freal<PB> r;
+  if (h < 0)
+    {
+      r.set_zero ();
+      st<PB> (r, Cl, Cse, nC, i);
+      return;
+    }
+  constexpr int WL = KWD + 2;
+  u32 win[WL];
+  u32 sticky = 0;
+#pragma unroll
+  for (int q = 0; q < WL; q++)
+    {
+      int s = h - (WL - 1) + q;
+      win[q] = (s >= 0) ? (u32) wbuf[s] : 0u;
+    }
+  for (int s = 0; s < h - (WL - 1); s++) sticky |= (u32) wbuf[s];
+  const u32 lz = (u32) cu_fp::core::clz32 (win[WL - 1]);
+#pragma unroll
+  for (int q = WL - 1; q >= 0; q--)
+    {
+      u32 lo = (q >= 1) ? win[q - 1] : 0u;
+      win[q] = cu_fp::core::fshl (win[q], lo, lz);
+    }
+  u32 K[KWD];
+  u32 co = cu_fp::core::round_top<KWD, WL> (K, win, 0u, sticky);
+  r.set_kept (K);
+  r.sign = neg ? -1 : 1;
+  r.exp = (long) (scale + 32 * (i64) (h + 1) - (i64) lz + (i64) co);
+  st<PB> (r, Cl, Cse, nC, i);
+}
+
+/* The MFMA kernel: one wave per 16x16 output tile (persistent over tiles).
+ * Lane l owns outputs (row 4*(l>>4)+v, col l&15), v < 4 (the 16x16 MFMA
+ * accumulator layout); its carry-save slots live in `slotbuf`. */
+template <int G>
+__global__ __launch_bounds__ (64) void k_exact_mfma (i64 m, i64 n, i64 mp, i64 np, i64 kp, int nG, int slots,
+                                                     const int8_t *__restrict__ DA, const int8_t *__restrict__ DB,
+                                                     const i64 *__restrict__ emaxA, const i64 *__restrict__ emaxB,
+                                                     i64 sA, i64 sB, i64 *__restrict__ slotbuf,
+                                                     u64 *__restrict__ Cl, i64 *__restrict__ Cse)
+{
+  const int l = threadIdx.x, r0 = l & 15, g = l >> 4;
+  const i64 tn = np / 16, tiles = (mp / 16) * tn;
+  i64 *my = slotbuf + ((i64) blockIdx.x * 64 + l) * 4 * slots;   /* 4 elements x slots */
+  for (i64 tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    {
+      const i64 ti = tile / tn, tj = tile % tn;
+      for (int s = 0; s < 4 * slots; s++) my[s] = 0;
+      const int8_t *pa = DA + (ti * 16 + r0) * kp + 16 * g;
+      const int8_t *pb = DB + (tj * 16 + r0) * kp + 16 * g;
+      const i64 pla = mp * kp, plb = np * kp;
+      for (int ga = 0; ga < nG; ga++)
+        for (int gb = 0; gb < nG; gb++)
+          for (i64 kc = 0; kc < kp; kc += EXACT_KCHUNK)
+            {
+              v4i acc[2 * G - 1];
+#pragma unroll
+              for (int d = 0; d < 2 * G - 1; d++) acc[d] = (v4i) {0, 0, 0, 0};
+              const i64 kend = (kc + EXACT_KCHUNK < kp) ? kc + EXACT_KCHUNK : kp;
+              for (i64 k0 = kc; k0 < kend; k0 += 64)
+                {
+                  v4i fa[G], fb[G];
+#pragma unroll
+                  for (int t = 0; t < G; t++)
+                    fa[t] = *(const v4i *) (pa + (i64) (ga * G + t) * pla + k0);
+#pragma unroll
+                  for (int u = 0; u < G; u++)
+                    fb[u] = *(const v4i *) (pb + (i64) (gb * G + u) * plb + k0);
+#pragma unroll
+                  for (int t = 0; t < G; t++)
+#pragma unroll
+                    for (int u = 0; u < G; u++)
+                      acc[t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8 (fa[t], fb[u], acc[t + u], 0, 0, 0);
+                }
+              /* flush: diagonal (ga+gb)*G + d sits at bit 8*diag */
+#pragma unroll
+              for (int d = 0; d < 2 * G - 1; d++)
+                {
+                  const int bit = 8 * ((ga + gb) * G + d);
+                  const int s = bit >> 5, sh = bit & 31;
+#pragma unroll
+                  for (int v = 0; v < 4; v++) my[v * slots + s] += (i64) acc[d][v] << sh;
+                }
+            }
+      /* carry-normalise each element's slots into 32-bit words and round */
+#pragma unroll 1
+      for (int v = 0; v < 4; v++)
+        {
+          const i64 i = ti * 16 + 4 * g + v, j = tj * 16 + r0;
+          i64 *sl = my + v * slots;
+          i64 carry = 0;
+          for (int s = 0; s < slots; s++)
+            {
+              i64 x = sl[s] + carry;
+              sl[s] = (i64) (u32) x;
+              carry = x >> 32;
+            }
+          if (i < m && j < n)
+            {
+              const i64 ea = emaxA[i], eb = emaxB[j];
+              const i64 scale = (ea == EXACT_EMPTY || eb == EXACT_EMPTY) ? 0 : ea + eb - sA - sB - 2 * (i64) PB;
+              round_store_int (sl, slots, scale, Cl, Cse, m * n, i * n + j);
+            }
+        }
+    }
+}
+
+/* VALU fallback: one lane per output, the exact sum accumulated in `nw`
+ * two's complement 32-bit words (one per 64-bit workspace slot) whose LSB
+ * weighs 2^(eminA_i + eminB_j - 2pb).  Rows r0 .. r0+rows-1 of C. */
+__global__ __launch_bounds__ (64) void k_exact_fallback (i64 m, i64 n, i64 k, i64 r0, i64 rows,
+                                                         const u64 *__restrict__ Al, const i64 *__restrict__ Ase,
+                                                         const u64 *__restrict__ Bl, const i64 *__restrict__ Bse,
+                                                         const i64 *__restrict__ eminA, const i64 *__restrict__ eminB,
+                                                         int nw, i64 *__restrict__ wsp, u64 *__restrict__ Cl,
+                                                         i64 *__restrict__ Cse)
+{
+  const i64 e = (i64) blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= rows * n) return;
+  const i64 i = r0 + e / n, j = e % n;
+  i64 *acc = wsp + e * nw;
+  for (int s = 0; s < nw; s++) acc[s] = 0;
+  const i64 base = (eminA[i] == EXACT_EMPTY || eminB[j] == EXACT_EMPTY) ? 0 : eminA[i] + eminB[j];
+  bool any = false;
+  for (i64 kk = 0; kk < k; kk++)
+    {
+      const i64 ia = i * k + kk, ib = kk * n + j;
+      const i64 wa = Ase[ia], wb = Bse[ib];
+      if ((wa >> 1) == MPGFX_ZERO_EXP || (wb >> 1) == MPGFX_ZERO_EXP) continue;
+      any = true;
+      u32 MA[KWD], MB[KWD], P[2 * KWD];
+      load_sig (Al, m * k, ia, MA);
+      load_sig (Bl, k * n, ib, MB);
+      cu_fp::core::mul_words<KWD> (P, MA, MB);
+      const i64 off = (wa >> 1) + (wb >> 1) - base;      /* >= 0 */
+      const int ws = (int) (off >> 5);
+      const u32 bs = (u32) (off & 31);
+      const bool sub = ((wa ^ wb) & 1) != 0;
+      u32 c = sub ? 1u : 0u;                             /* two's complement add of +-P<<off */
+      const u32 xm = sub ? 0xffffffffu : 0u;
+      int s = ws;
+#pragma unroll
+      for (int q = 0; q <= 2 * KWD; q++, s++)
+        {
+          u32 hi = (q < 2 * KWD) ? P[q] : 0u, lo = (q >= 1) ? P[q - 1] : 0u;
+          u32 word = cu_fp::core::fshl (hi, lo, bs) ^ xm;
+          if (s < nw) acc[s] = (i64) cu_fp::core::addc ((u32) acc[s], word, c);
+        }
+      /* words above the shifted product: 0 (add) or all ones (subtract) */
+      for (; s < nw; s++)
+        {
+          if (!sub && !c) break;
+          acc[s] = (i64) cu_fp::core::addc ((u32) acc[s], xm, c);
+        }
+      /* below the product a subtraction adds ~0 words with the +1 carry:
+       * ~0 + 1 = 0 with carry 1 -- already folded into the carry-in above */
+    }
+  if (!any)
+    {
+      cu_freal<PB> z;
+      z.set_zero ();
+      st<PB> (z, Cl, Cse, m * n, i * n + j);
+      return;
+    }
+  round_store_int (acc, nw, base - 2 * (i64) PB, Cl, Cse, m * n, i * n + j);
+}
+
+int exact_digits_launch (i64 lines, i64 len, i64 ls, i64 ps, const u64 *l, const i64 *se, i64 nv, const i64 *emax,
+                         i64 spread, int S, int Spad, i64 Lpad, i64 kp, int8_t *D, hipStream_t s)
+{
+  dim3 grid ((unsigned) ((kp + 255) / 256), (unsigned) Lpad);
+  hipLaunchKernelGGL (k_exact_digits, grid, dim3 (256), 0, s, lines, len, ls, ps, l, se, nv, emax, spread, S, Spad,
+                      Lpad, kp, D);
+  return launch_status ("gemm_exact digits");
+}
+
+int exact_mfma_launch (int G, const exact_plan &p, const int8_t *DA, const int8_t *DB, const i64 *emaxA,
+                       const i64 *emaxB, i64 *slotbuf, int grid, u64 *Cl, i64 *Cse, hipStream_t s)
+{
+#define EXACT_G(GG)                                                                                        \
+  if (G == GG)                                                                                             \
+    {                                                                                                      \
+      hipLaunchKernelGGL ((k_exact_mfma<GG>), dim3 (grid), dim3 (64), 0, s, p.m, p.n, p.mp, p.np, p.kp,    \
+                          p.nG, p.slots, DA, DB, emaxA, emaxB, p.sA, p.sB, slotbuf, Cl, Cse);              \
+      return launch_status ("gemm_exact mfma");                                                            \
+    }
+  EXACT_G (8)
+#undef EXACT_G
+  return MPGFX_EINVAL;
+}
+
+int exact_fallback_launch (i64 m, i64 n, i64 k, i64 r0, i64 rows, const u64 *Al, const i64 *Ase, const u64 *Bl,
+                           const i64 *Bse, const i64 *eminA, const i64 *eminB, int nw, i64 *wsp, u64 *Cl, i64 *Cse,
+                           hipStream_t s)
+{
+  i64 tot = rows * n;
+  hipLaunchKernelGGL (k_exact_fallback, dim3 ((unsigned) ((tot + 63) / 64)), dim3 (64), 0, s, m, n, k, r0, rows, Al,
+                      Ase, Bl, Bse, eminA, eminB, nw, wsp, Cl, Cse);
+  return launch_status ("gemm_exact fallback");
+}

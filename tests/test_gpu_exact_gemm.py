@@ -1,0 +1,93 @@
+"""Exact-dot GEMM (INT8-MFMA slice products, one rounding per output) against
+the MPFR golden fixtures (exact products + mpfr_sum) and the oracle's exact
+sum, including the VALU wide-accumulator path for exponent spreads beyond the
+MFMA window."""
+import numpy as np
+import pytest
+
+import mpc_cuda_amd as M
+from mpc_cuda_amd.arrays import MPArray
+from oracle import oracle as O
+import goldens
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(pair, pb):
+    return MPArray(pb, np.ascontiguousarray(pair[0]), np.ascontiguousarray(pair[1])).cuda()
+
+
+@pytest.mark.parametrize("pb", goldens.PBS)
+def test_exact_gemm_vs_golden(gpu, pb):
+    g = goldens.load(pb)
+    m, n, k = (int(v) for v in g["dims"])
+    for tag in ("g4", "g80"):
+        A, B = goldens.pair(g, tag + "A"), goldens.pair(g, tag + "B")
+        r = M.gemm(dev(A, pb), dev(B, pb), m, n, k, "exact")
+        assert O.mismatches(r.pair(), goldens.pair(g, tag + "exact"), ignore_zero_sign=True) == 0, tag
+        assert O.mismatches(r.pair(), O.gemm(pb, m, n, k, A, B, "exact"), ignore_zero_sign=False) == 0, tag
+
+
+@pytest.mark.parametrize("pb,m,n,k,lo,hi", [
+    (256, 33, 17, 100, -4, 4),      # ragged tiles and k not a multiple of 64
+    (256, 40, 40, 64, -80, 80),     # wide spread, still inside the MFMA window
+    (64, 50, 30, 300, -20, 20),
+    (512, 20, 24, 129, -4, 4),
+    (96, 18, 19, 70, -40, 40),      # sub-limb precision (SB = 32)
+    (1024, 16, 16, 65, -16, 16),
+])
+def test_exact_gemm_vs_oracle(gpu, pb, m, n, k, lo, hi):
+    A = M.random_real(pb, m * k, 11 + pb + k, lo, hi)
+    B = M.random_real(pb, k * n, 12 + pb + k, lo, hi)
+    r = M.gemm(A.cuda(), B.cuda(), m, n, k, "exact")
+    assert O.mismatches(r.pair(), O.gemm(pb, m, n, k, A.pair(), B.pair(), "exact"), ignore_zero_sign=False) == 0
+
+
+def test_exact_gemm_cancellation_and_zeros(gpu):
+    """Rows whose products cancel exactly give +0; zero entries and an all-zero
+    row are skipped; a tiny term that survives a huge cancellation is exact."""
+    pb, m, n, k = 256, 4, 3, 6
+    A = M.random_real(pb, m * k, 5, -4, 4)
+    B = M.random_real(pb, k * n, 6, -4, 4)
+    Al, Ase = A.limbs.copy(), A.se.copy()
+    # row 0: [x, x] against column pairs that negate -> exact zero per column
+    Al[:, 1] = Al[:, 0]; Ase[1] = Ase[0] ^ 1
+    Bl, Bse = B.limbs.copy(), B.se.copy()
+    for j in range(n):
+        Bl[:, 1 * n + j] = Bl[:, 0 * n + j]; Bse[1 * n + j] = Bse[0 * n + j]
+    Ase[2:6] = M.ZERO_EXP * 2; Al[:, 2:6] = 0          # rest of row 0 zero
+    Ase[6:12] = M.ZERO_EXP * 2; Al[:, 6:12] = 0        # row 1 all zero
+    A2 = MPArray(pb, Al, Ase)
+    B2 = MPArray(pb, Bl, Bse)
+    r = M.gemm(A2.cuda(), B2.cuda(), m, n, k, "exact")
+    ref = O.gemm(pb, m, n, k, A2.pair(), B2.pair(), "exact")
+    assert O.mismatches(r.pair(), ref, ignore_zero_sign=False) == 0
+    h = r.numpy()
+    assert np.all((h.se[:2 * n] >> 1) == M.ZERO_EXP)
+
+
+def test_exact_gemm_fallback_extreme_spread(gpu):
+    """Exponent spreads of thousands of bits exceed the MFMA window (64 digits)
+    and take the VALU wide-accumulator path; results stay exact."""
+    pb, m, n, k = 128, 6, 5, 40
+    A = M.random_real(pb, m * k, 21, -3000, 3000)
+    B = M.random_real(pb, k * n, 22, -3000, 3000)
+    r = M.gemm(A.cuda(), B.cuda(), m, n, k, "exact")
+    assert O.mismatches(r.pair(), O.gemm(pb, m, n, k, A.pair(), B.pair(), "exact"), ignore_zero_sign=False) == 0
+
+
+@pytest.mark.parametrize("lo,hi", [(-4, 4), (-80, 80)])
+def test_exact_gemm_256_full_size_sampled(gpu, lo, hi):
+    """The 1024^3 pb-256 configuration (both exponent sets): the oracle's exact
+    sum recomputes 8 full rows (8192 outputs) and they must match bit for bit;
+    the per-op GEMM of the same inputs differs from the exact one somewhere."""
+    pb, m, n, k = 256, 1024, 1024, 1024
+    A = M.random_real(pb, m * k, 501 + hi, lo, hi)
+    B = M.random_real(pb, k * n, 502 + hi, lo, hi)
+    C = M.gemm(A.cuda(), B.cuda(), m, n, k, "exact").numpy()
+    Ap, Bp = A.pair(), B.pair()
+    for r0 in (0, 77, 255, 400, 640, 777, 999, m - 1):
+        ref = O.gemm(pb, m, n, k, Ap, Bp, "exact", rows=(r0, r0 + 1))
+        sl = slice(r0 * n, (r0 + 1) * n)
+        assert np.array_equal(C.limbs[:, sl], ref[0][:, sl]), r0
+        assert np.array_equal(C.se[sl], ref[1][sl]), r0
