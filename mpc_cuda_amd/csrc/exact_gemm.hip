@@ -106,7 +106,7 @@ int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, con
   unsigned long long *spreads = (unsigned long long *) (eminB + n);
   int rc = MPGFX_OK;
   auto done = [&] (int r) {
-    if (owned) hipFreeAsync (owned, s);
+    if (owned) (void) hipFreeAsync (owned, s);
     return r;
   };
   if ((rc = hip_ok ("gemm_exact memset", hipMemsetAsync (spreads, 0, 2 * sizeof (unsigned long long), s)))) return done (rc);
@@ -165,7 +165,7 @@ int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, con
       i64 nr = (r0 + rows <= m) ? rows : m - r0;
       rc = o->exact_fallback (m, n, k, r0, nr, Al, Ase, Bl, Bse, eminA, eminB, nw, wsp, Cl, Cse, s);
     }
-  hipFreeAsync (wsp, s);
+  (void) hipFreeAsync (wsp, s);
   return done (rc);
 }
 
