@@ -146,6 +146,68 @@ int mpgfx_pack (int pb, int complex, int64_t n, const void *aos,
 int mpgfx_unpack (int pb, int complex, int64_t n, const uint64_t *limbs,
                   const int64_t *se, void *aos, void *stream);
 
+/* n seeded synthetic nonzero values (value i = start + i of the stream keyed by
+ * seed: significand limbs random with the top bit set and the low 64N-pb bits
+ * clear, random sign, exponent uniform in [emin, emax]) -- the generator of the
+ * tests and benchmarks, identical to mpc_cuda_amd/gen.py.  No counterpart in
+ * the reference (its tests draw operands with xorshift64 on the host,
+ * tools/test_freal.cu:15-27). */
+int mpgfx_fill_random (int pb, int64_t n, int64_t start, uint64_t seed, int64_t emin, int64_t emax,
+                       uint64_t *limbs, int64_t *se, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Batched execution (no counterpart in the reference).  Many AXPY, dot and
+ * GEMV instances of uneven lengths, mixed precision and real/complex run as one
+ * hipGraph: per (precision, real/complex) class a persistent product kernel
+ * (all AXPY outputs and every product of every dot and GEMV row, in parallel)
+ * followed by a persistent fold kernel (the per-operation k-order additions,
+ * one lane per dot / GEMV row, longest chains first).  Dot and GEMV semantics
+ * are the reference's per-op fold (tools/linalg_fast.cu:37-51): acc = +0,
+ * acc = RNDN(acc + RNDN(x_k*y_k)) for k ascending (complex: MPC_RNDNN).
+ * ------------------------------------------------------------------------ */
+enum mpgfx_task_kind { MPGFX_TASK_AXPY = 0, MPGFX_TASK_DOT = 1, MPGFX_TASK_GEMV = 2 };
+
+/* One instance; all pointers are device MP arrays of precision pb (complex
+ * layout when cplx != 0).
+ *   AXPY: out[i] = in0 * in1[i] + in2[i], i < n; in0 is ONE value.
+ *   DOT : out[0] = fold_k in0[k] * in1[k], k < n.
+ *   GEMV: out[i] = fold_k in0[i*n + k] * in1[k], i < m, k < n (in0: m x n). */
+typedef struct mpgfx_task
+{
+  int32_t kind, pb, cplx, reserved;
+  int64_t m, n;
+  const uint64_t *in0_limbs; const int64_t *in0_se;
+  const uint64_t *in1_limbs; const int64_t *in1_se;
+  const uint64_t *in2_limbs; const int64_t *in2_se;
+  uint64_t *out_limbs; int64_t *out_se;
+} mpgfx_task;
+
+/* Mapping of one (phase, precision, real/complex) class: values per work item
+ * of the product phase, lanes per workgroup, persistent workgroups per CU. */
+typedef struct mpgfx_mapping
+{
+  int32_t chunk, block, wg_per_cu, reserved;
+} mpgfx_mapping;
+
+typedef struct mpgfx_batch mpgfx_batch;
+
+/* Plan a batch: validate the `ntasks` host-side descriptors, allocate the
+ * device work lists and product scratch (sized for the batch), and capture
+ * the launches into a hipGraph on `stream`.  tuning: 0 = fixed default
+ * mapping, 1 = the tuner's mapping (read from the JSON cache file `cache`,
+ * or measured on this batch and written there when absent; cache may be NULL
+ * to tune without a file).  The batch keeps its own copy of the descriptors. */
+int mpgfx_batch_create (const mpgfx_task *tasks, int64_t ntasks, int tuning, const char *cache,
+                        void *stream, mpgfx_batch **out);
+
+/* Launch the whole batch (one graph launch) on `stream`. */
+int mpgfx_batch_run (mpgfx_batch *b, void *stream);
+
+/* Bytes of device scratch the batch holds, and the number of classes. */
+int mpgfx_batch_info (const mpgfx_batch *b, size_t *scratch_bytes, int *classes);
+
+int mpgfx_batch_destroy (mpgfx_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

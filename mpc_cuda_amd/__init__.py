@@ -9,10 +9,11 @@ fallback: every operation runs the HIP kernels or raises.
 """
 from . import _lib
 from .arrays import MPArray, empty, equal, nlimbs, ZERO_EXP
-from .gen import random_real, random_complex, random_array
+from .gen import random_real, random_complex, random_array, device_random
+from . import batch
 from .ops import elementwise, axpy, gemv, gemm, gemm_workspace_size, supported_pb
 from ._lib import MPGFXError, lib
 
 __all__ = ["MPArray", "empty", "equal", "nlimbs", "ZERO_EXP", "random_real", "random_complex",
-           "random_array", "elementwise", "axpy", "gemv", "gemm", "gemm_workspace_size",
+           "random_array", "device_random", "batch", "elementwise", "axpy", "gemv", "gemm", "gemm_workspace_size",
            "supported_pb", "MPGFXError", "lib"]

@@ -150,3 +150,59 @@ extern "C" int mpgfx_unpack (int pb, int cplx, int64_t n, const uint64_t *l, con
   NEED (aos && l && se);
   return o->unpack (cplx ? 1 : 0, n, l, se, aos, (hipStream_t) stream);
 }
+
+/* ---- seeded synthetic values (same stream as mpc_cuda_amd/gen.py) ---- */
+namespace mpgfx {
+
+__device__ __forceinline__ u64 splitmix64 (u64 x)
+{
+  x += 0x9E3779B97F4A7C15ull;
+  u64 z = x;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ u64 xs64 (u64 s)
+{
+  s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+  return s;
+}
+
+/* value i (of n, starting at `start`): stream s = splitmix64(i ^ splitmix64(seed)),
+ * then xorshift64 draws: sign, nl limbs (top bit forced, low sb bits cleared),
+ * exponent = draw % (emax-emin+1) + emin */
+__global__ void k_fill_random (i64 n, i64 start, u64 seed, int nl, int sb, i64 emin, i64 emax, u64 *l, i64 *se)
+{
+  i64 i = (i64) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u64 s = splitmix64 ((u64) (start + i) ^ splitmix64 (seed));
+  if (s == 0) s = 1;
+  s = xs64 (s);
+  i64 neg = (i64) (s & 1);
+  for (int q = 0; q < nl; q++)
+    {
+      s = xs64 (s);
+      u64 v = s;
+      if (q == nl - 1) v |= 1ull << 63;
+      if (q == 0 && sb) v &= ~((1ull << sb) - 1);
+      l[(i64) q * n + i] = v;
+    }
+  s = xs64 (s);
+  u64 span = (u64) (emax - emin + 1);
+  i64 e = (i64) (s % span) + emin;
+  se[i] = (i64) ((u64) e << 1) | neg;
+}
+
+} // namespace mpgfx
+
+extern "C" int mpgfx_fill_random (int pb, int64_t n, int64_t start, uint64_t seed, int64_t emin, int64_t emax,
+                                  uint64_t *limbs, int64_t *se, void *stream)
+{
+  if (n < 0 || emax < emin || pb <= 0 || pb % 32) return MPGFX_EINVAL;
+  if (n == 0) return MPGFX_OK;
+  if (!limbs || !se) return MPGFX_EINVAL;
+  int nl = (pb + 63) / 64, sb = 64 * nl - pb;
+  hipLaunchKernelGGL (mpgfx::k_fill_random, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, (hipStream_t) stream, n,
+                      start, seed, nl, sb, emin, emax, limbs, se);
+  return launch_status ("mpgfx_fill_random");
+}

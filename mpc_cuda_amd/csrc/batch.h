@@ -1,0 +1,59 @@
+/* batch.h -- device-side structures of the batched persistent execution
+ * (mpgfx_batch_*, include/mpgfx.h).
+ *
+ * One class = all tasks of one (precision, real/complex).  Its work:
+ *  phase 1  `items`: contiguous element ranges of one task -- AXPY outputs,
+ *           or products of a dot / GEMV, each written (rounded, as the per-op
+ *           fold needs) into the product scratch;
+ *  phase 2  `chains`: one per dot and per GEMV row, sorted by length
+ *           (longest first) and grouped by 64; a wavefront folds one group, one
+ *           chain per lane.  Step s of the chain at position p lives at scratch
+ *           index gbase[p/64] + 64*s + p%64, so a wavefront's loads at step s
+ *           are contiguous.
+ */
+#ifndef MPGFX_BATCH_H
+#define MPGFX_BATCH_H
+
+#include "common.h"
+
+namespace mpgfx {
+
+struct bt_task
+{
+  int32_t kind, cplx;
+  i64 m, n;
+  const u64 *in0l; const i64 *in0s;
+  const u64 *in1l; const i64 *in1s;
+  const u64 *in2l; const i64 *in2s;
+  u64 *outl; i64 *outs;
+  i64 p0;                                   /* first chain position (dot / GEMV) */
+};
+
+struct bt_item
+{
+  int32_t task, count;
+  i64 start;
+};
+
+struct bt_chain
+{
+  int32_t task, row;                        /* task < 0: padding lane */
+  i64 len;
+};
+
+struct bt_class
+{
+  const bt_task *tasks;
+  const bt_item *items;
+  i64 nitems;
+  const bt_chain *chains;
+  const i64 *gbase;
+  i64 ngroups;
+  u64 *scr_l;                               /* product scratch MP array */
+  i64 *scr_s;
+  i64 scr_n;                                /* values (per part when complex) */
+  unsigned *counters;                       /* [0] phase-1 queue, [1] phase-2 queue */
+};
+
+} // namespace mpgfx
+#endif

@@ -4,6 +4,7 @@
 
 #include "common.h"
 #include "exact.h"
+#include "batch.h"
 
 namespace mpgfx {
 
@@ -31,6 +32,8 @@ struct pb_ops
   int (*exact_fallback) (i64 m, i64 n, i64 k, i64 r0, i64 rows, const u64 *Al, const i64 *Ase, const u64 *Bl,
                          const i64 *Bse, const i64 *eminA, const i64 *eminB, int nw, i64 *wsp, u64 *Cl, i64 *Cse,
                          hipStream_t);
+  /* batched persistent kernels (batch_pb.h): phase 0 reset, 1 products, 2 folds */
+  int (*batch_launch) (int phase, int cplx, const bt_class &c, int grid, int block, hipStream_t);
 };
 
 #define MPGFX_DECL(P) const pb_ops *ops_pb_##P ();

@@ -62,3 +62,21 @@ def random_complex(pb, n, seed, emin=-8, emax=8):
 
 def random_array(pb, n, seed, emin=-8, emax=8, cplx=False):
     return (random_complex if cplx else random_real)(pb, n, seed, emin, emax)
+
+
+def device_random(pb, n, seed, emin=-8, emax=8, cplx=False, device="cuda"):
+    """The same values as random_real / random_complex, generated on the GPU
+    by mpgfx_fill_random."""
+    import ctypes
+    import torch
+    from . import _lib
+    from .arrays import empty
+    out = empty(pb, n, cplx, device=device)
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    L = _lib.lib()
+    parts = [(out.limbs, out.se, seed)] if not cplx else [
+        (out.limbs[0], out.se[0], seed), (out.limbs[1], out.se[1], seed ^ 0x5DEECE66D)]
+    for l, se, sd in parts:
+        _lib.check("mpgfx_fill_random", L.mpgfx_fill_random(pb, n, 0, sd & 0xFFFFFFFFFFFFFFFF, emin, emax,
+                                                            l.data_ptr(), se.data_ptr(), s))
+    return out
