@@ -405,12 +405,17 @@ extern "C" int mpgfx_batch_create (const mpgfx_task *tasks, int64_t ntasks, int 
         if (!rc) rc = tune_class (c, b->cus, s, cmap, &dirty);
       if (!rc && dirty) write_cache (cache, cmap);
     }
-  /* ---- graph: per class [reset -> products -> folds] on its own branch ---- */
+  /* ---- graph: per class [reset -> products -> folds] on its own branch,
+   * captured on a private stream (the caller's may be the legacy null stream,
+   * which cannot be captured); the graph is launched on the caller's stream ---- */
+  if (!rc) rc = hip_ok ("batch stream sync", hipStreamSynchronize (s));
   if (!rc)
     {
-      hipEvent_t fork, *joins = nullptr;
-      std::vector<hipEvent_t> jv (b->classes.size ());
-      rc = hip_ok ("batch event", hipEventCreateWithFlags (&fork, hipEventDisableTiming));
+      hipStream_t cs = nullptr;
+      hipEvent_t fork = nullptr;
+      std::vector<hipEvent_t> jv (b->classes.size (), nullptr);
+      rc = hip_ok ("batch capture stream", hipStreamCreateWithFlags (&cs, hipStreamNonBlocking));
+      if (!rc) rc = hip_ok ("batch event", hipEventCreateWithFlags (&fork, hipEventDisableTiming));
       for (size_t i = 0; !rc && i < b->classes.size (); i++)
         {
           hipStream_t ss;
@@ -418,28 +423,28 @@ extern "C" int mpgfx_batch_create (const mpgfx_task *tasks, int64_t ntasks, int 
           if (!rc) b->side.push_back (ss);
           if (!rc) rc = hip_ok ("batch event", hipEventCreateWithFlags (&jv[i], hipEventDisableTiming));
         }
-      (void) joins;
-      if (!rc) rc = hip_ok ("batch capture", hipStreamBeginCapture (s, hipStreamCaptureModeThreadLocal));
+      if (!rc) rc = hip_ok ("batch capture", hipStreamBeginCapture (cs, hipStreamCaptureModeThreadLocal));
       if (!rc)
         {
-          int lrc = hip_ok ("batch fork", hipEventRecord (fork, s));
+          int lrc = hip_ok ("batch fork", hipEventRecord (fork, cs));
           for (size_t i = 0; !lrc && i < b->classes.size (); i++)
             {
               lrc = hip_ok ("batch fork wait", hipStreamWaitEvent (b->side[i], fork, 0));
               if (!lrc) lrc = launch_class (b->classes[i], b->cus, b->side[i]);
               if (!lrc) lrc = hip_ok ("batch join", hipEventRecord (jv[i], b->side[i]));
-              if (!lrc) lrc = hip_ok ("batch join wait", hipStreamWaitEvent (s, jv[i], 0));
+              if (!lrc) lrc = hip_ok ("batch join wait", hipStreamWaitEvent (cs, jv[i], 0));
             }
           hipGraph_t g = nullptr;
-          int erc = hip_ok ("batch end capture", hipStreamEndCapture (s, &g));
+          int erc = hip_ok ("batch end capture", hipStreamEndCapture (cs, &g));
           rc = lrc ? lrc : erc;
           if (!rc) b->graph = g;
           else if (g) (void) hipGraphDestroy (g);
         }
       if (!rc) rc = hip_ok ("batch instantiate", hipGraphInstantiate (&b->exec, b->graph, nullptr, nullptr, 0));
-      (void) hipEventDestroy (fork);
+      if (fork) (void) hipEventDestroy (fork);
       for (auto &e : jv)
         if (e) (void) hipEventDestroy (e);
+      if (cs) (void) hipStreamDestroy (cs);
     }
   if (rc)
     {
