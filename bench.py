@@ -98,6 +98,8 @@ def main():
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(a, x, y, z)
 
+    rec["side_metrics"] = side_metrics()
+
     if args.extra:
         rec["extra"] = extra_configs(args)
 
@@ -129,44 +131,147 @@ def cpu_baseline(a, x, y, z):
             "gpu_vs_cpu_mismatches": int(bad)}
 
 
-def extra_configs(args):
+def _timed(fn, steps):
     import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def side_metrics():
+    """The other halves of the metric, measured in the same run: real AXPY at
+    256 bits (n = 2^20) and the 1024^3 GEMM at 256 bits, per-op vs MFMA
+    exact-dot (exponents in [-4, 4])."""
     import mpc_cuda_amd as M
     out = {}
+    pb, n = 256, 1 << 20
+    a = M.device_random(pb, 1, 101, -8, 8)
+    x = M.device_random(pb, n, 102, -8, 8)
+    y = M.device_random(pb, n, 103, -8, 8)
+    o = M.empty(pb, n)
+    ms = _timed(lambda: M.axpy(a, x, y, out=o), 20)
+    out["axpy_real_pb256_n2^20"] = {"ms": ms, "MP_ops_per_s": 2 * n / (ms * 1e-3),
+                                    "GB_per_s": 120 * n / (ms * 1e-3) / 1e9, "hbm_frac": 120 * n / (ms * 1e-3) / 8e12}
+    pb, d = 256, 1024
+    A = M.device_random(pb, d * d, 401, -4, 4)
+    B = M.device_random(pb, d * d, 402, -4, 4)
+    C = M.empty(pb, d * d)
+    ms_p = _timed(lambda: M.gemm(A, B, d, d, d, "perop", out=C), 1)
+    ms_e = _timed(lambda: M.gemm(A, B, d, d, d, "exact", out=C), 2)
+    out["gemm_pb256_1024^3"] = {"perop_ms": ms_p, "exact_ms": ms_e, "exact_speedup": ms_p / ms_e,
+                                "perop_MP_ops_per_s": 2 * d ** 3 / (ms_p * 1e-3),
+                                "exact_MP_ops_per_s": 2 * d ** 3 / (ms_e * 1e-3)}
+    return out
 
-    def timed(fn, steps):
-        fn()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(steps):
-            fn()
-        e1.record()
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / steps
 
-    for pb in (64, 128, 256, 512, 2048):
+def extra_configs(args):
+    """Every other configuration, on GPU-generated inputs: real AXPY over the
+    precision sweep (n = 2^20), GEMV 4096^2 at 512 bits, the exact GEMM with
+    exponent spread [-80, 80], and the 10,000-instance batched workload."""
+    import mpc_cuda_amd as M
+    out = {}
+    for pb in (64, 128, 512, 1024, 2048):
         n = 1 << 20
-        a = M.random_real(pb, 1, 1, -8, 8).cuda()
-        x = M.random_real(pb, n, 2, -8, 8).cuda()
-        y = M.random_real(pb, n, 3, -8, 8).cuda()
+        a = M.device_random(pb, 1, 1, -8, 8)
+        x = M.device_random(pb, n, 2, -8, 8)
+        y = M.device_random(pb, n, 3, -8, 8)
         o = M.empty(pb, n)
-        ms = timed(lambda: M.axpy(a, x, y, out=o), 10)
+        ms = _timed(lambda: M.axpy(a, x, y, out=o), 10)
         out[f"axpy_real_pb{pb}"] = {"ms": ms, "MP_ops_per_s": 2 * n / (ms * 1e-3),
                                     "GB_per_s": 3 * (pb // 8 + 8) * n / (ms * 1e-3) / 1e9,
                                     "word_products_per_s": (pb // 32) ** 2 * n / (ms * 1e-3)}
+        del a, x, y, o
     pb, m, k = 512, 4096, 4096
-    A = M.random_real(pb, m * k, 301, -4, 4).cuda()
-    xv = M.random_real(pb, k, 302, -4, 4).cuda()
-    ms = timed(lambda: M.gemv(A, xv, m, k), 2)
+    A = M.device_random(pb, m * k, 301, -4, 4)
+    xv = M.device_random(pb, k, 302, -4, 4)
+    ms = _timed(lambda: M.gemv(A, xv, m, k), 2)
     out["gemv_pb512_4096"] = {"ms": ms, "MP_ops_per_s": 2 * m * k / (ms * 1e-3),
                               "word_products_per_s": m * k * (pb // 32) ** 2 / (ms * 1e-3)}
-    pb, n = 256, 1024
-    A = M.random_real(pb, n * n, 401, -4, 4).cuda()
-    B = M.random_real(pb, n * n, 402, -4, 4).cuda()
-    ms = timed(lambda: M.gemm(A, B, n, n, n, "perop"), 1)
-    out["gemm_perop_pb256_1024"] = {"ms": ms, "MP_ops_per_s": 2 * n ** 3 / (ms * 1e-3)}
+    del A, xv
+    pb, d = 256, 1024
+    A = M.device_random(pb, d * d, 501, -80, 80)
+    B = M.device_random(pb, d * d, 502, -80, 80)
+    ms = _timed(lambda: M.gemm(A, B, d, d, d, "exact"), 2)
+    out["gemm_exact_pb256_1024^3_spread80"] = {"ms": ms, "MP_ops_per_s": 2 * d ** 3 / (ms * 1e-3)}
+    del A, B
+    out["batch_10000"] = batch_config(args)
     return out
+
+
+def batch_config(args, n_tasks=10000, seed=5, check=50):
+    """10,000 instances: AXPY / dot / GEMV at 40/40/20 %, half complex, lengths
+    log-uniform in [16, 65536], GEMV rows log-uniform in [16, 2048] with the
+    matrix capped at 2^18 values (columns reduced), precision uniform in
+    {64, 128, 256, 512, 1024}, exponents in [-8, 8]; one graph launch.  Fixed
+    mapping vs tuned mapping; `check` sampled instances are verified against
+    the oracle."""
+    import math
+    import random
+    import torch
+    import mpc_cuda_amd as M
+    from mpc_cuda_amd import batch as B
+    rng = random.Random(seed)
+    logu = lambda a, b: int(round(2 ** rng.uniform(math.log2(a), math.log2(b))))
+    tasks, keep, specs = [], [], []
+    mp_ops, wp = 0, 0
+    for t in range(n_tasks):
+        kind = rng.choices([B.AXPY, B.DOT, B.GEMV], [0.4, 0.4, 0.2])[0]
+        pb = rng.choice([64, 128, 256, 512, 1024])
+        cplx = rng.random() < 0.5
+        L = logu(16, 65536)
+        R = 1
+        if kind == B.GEMV:
+            R = logu(16, 2048)
+            L = max(1, min(L, (1 << 18) // R))
+        g = lambda cnt, s: M.device_random(pb, cnt, s, -8, 8, cplx)
+        sd = 7919 * t + 13
+        if kind == B.AXPY:
+            arrs = (g(1, sd), g(L, sd + 1), g(L, sd + 2), M.empty(pb, L, cplx))
+            tasks.append(B.axpy(*arrs))
+            elems = L
+        elif kind == B.DOT:
+            arrs = (g(L, sd), g(L, sd + 1), M.empty(pb, 1, cplx))
+            tasks.append(B.dot(*arrs))
+            elems = L
+        else:
+            arrs = (g(R * L, sd), g(L, sd + 1), M.empty(pb, R, cplx))
+            tasks.append(B.gemv(arrs[0], arrs[1], R, L, arrs[2]))
+            elems = R * L
+        specs.append((kind, pb, cplx, R, L, arrs))
+        mp_ops += 2 * elems
+        wp += elems * (pb // 32) ** 2 * (4 if cplx else 1)
+    torch.cuda.synchronize()
+    res = {"instances": n_tasks, "mp_ops": mp_ops, "word_products": wp}
+    for label, tuning in (("fixed", False), ("tuned", True)):
+        bt = B.Batch(tasks, tuning=tuning, cache=os.path.join(ROOT, "gpurun_out", "batch_tune.json") if tuning else None)
+        ms = _timed(bt.run, 3)
+        res[label] = {"ms": ms, "instances_per_s": n_tasks / (ms * 1e-3), "MP_ops_per_s": mp_ops / (ms * 1e-3),
+                      "word_products_per_s": wp / (ms * 1e-3), "scratch_GB": bt.info()["scratch_bytes"] / 1e9}
+        bt.close()
+    if check:
+        from oracle import oracle as O
+        bad = 0
+        for i in sorted(rng.sample(range(n_tasks), check)):
+            kind, pb, cplx, R, L, arrs = specs[i]
+            if kind == B.AXPY:
+                ref = (O.axpy_complex if cplx else O.axpy_real)(pb, arrs[0].pair(), arrs[1].pair(), arrs[2].pair())
+                got = arrs[3]
+            elif kind == B.DOT:
+                ref = O.gemm(pb, 1, 1, L, arrs[0].pair(), arrs[1].pair(), "cperop" if cplx else "perop")
+                got = arrs[2]
+            else:
+                ref = O.gemm(pb, R, 1, L, arrs[0].pair(), arrs[1].pair(), "cperop" if cplx else "perop")
+                got = arrs[2]
+            bad += O.mismatches(got.pair(), ref, ignore_zero_sign=False)
+        res["sampled_instances_checked"] = check
+        res["sampled_mismatches"] = bad
+    return res
 
 
 if __name__ == "__main__":
