@@ -62,15 +62,27 @@ CUFP_INL u32 addc (u32 a, u32 b, u32 &c)
   return (u32) t;
 }
 
-/* top 32 bits of (hi:lo) << s, s in [0,31] */
-CUFP_INL u32 fshl (u32 hi, u32 lo, u32 s)
-{
-  return (u32) (((((u64) hi << 32) | lo) << s) >> 32);
-}
+/* Funnel shifts.  On the device they are v_alignbit_b32; the 64-bit form is
+ * kept for the host only, because on the device the optimiser merges the
+ * (hi:lo) pair of two adjacent array words into one 64-bit load, which keeps
+ * the whole array out of registers. */
 /* low 32 bits of (hi:lo) >> s, s in [0,31] */
 CUFP_INL u32 fshr (u32 hi, u32 lo, u32 s)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_alignbit (hi, lo, s);
+#else
   return (u32) ((((u64) hi << 32) | lo) >> s);
+#endif
+}
+/* top 32 bits of (hi:lo) << s, s in [0,31] */
+CUFP_INL u32 fshl (u32 hi, u32 lo, u32 s)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return s ? __builtin_amdgcn_alignbit (hi, lo, 32u - s) : hi;
+#else
+  return (u32) (((((u64) hi << 32) | lo) << s) >> 32);
+#endif
 }
 
 CUFP_INL int clz32 (u32 x)
@@ -83,11 +95,20 @@ CUFP_INL int clz32 (u32 x)
 }
 
 /* ---------------------------------------------------------------------------
- * Exact product: p[2W] = a[W] * b[W] (schoolbook by rows; each step is one
- * 32x32+64 -> 64 multiply-add).
+ * Exact product p[2W] = a[W] * b[W] of two W-word significands.
+ *
+ * Small W: schoolbook by rows on 32-bit words (one 32x32+64 multiply-add per
+ * word product plus the carry add).
+ * W >= 8: the operands are re-cut into LB-bit limbs (LB = 29, or 28 for the
+ * widest precisions) and the product is accumulated by columns: a column of
+ * up to 63 (255) limb products plus the incoming carry cannot overflow 64 bits,
+ * so every limb product is exactly one v_mad_u64_u32 with the running column
+ * sum as its addend and no carry handling inside the column; each column then
+ * emits one LB-bit limb and shifts its carry into the next.  The limbs are cut
+ * back into 32-bit words at the end.
  * ------------------------------------------------------------------------- */
 template <int W>
-CUFP_INL void mul_words (u32 (&p)[2 * W], const u32 (&a)[W], const u32 (&b)[W])
+CUFP_INL void mul_words_rows (u32 (&p)[2 * W], const u32 (&a)[W], const u32 (&b)[W])
 {
   CUFP_UNROLL
   for (int i = 0; i < 2 * W; i++) p[i] = 0;
@@ -102,6 +123,70 @@ CUFP_INL void mul_words (u32 (&p)[2 * W], const u32 (&a)[W], const u32 (&b)[W])
     });
     p[W + j] = c;
   });
+}
+
+template <int W>
+struct limb_cut
+{
+  static constexpr int LB = (32 * W <= 29 * 63) ? 29 : 28;   /* column capacity 63 / 255 */
+  static constexpr int NL = (32 * W + LB - 1) / LB;
+  static constexpr u32 MASK = (1u << LB) - 1u;
+  static_assert (NL <= ((LB == 29) ? 63 : 255), "column sum could overflow");
+};
+
+/* bits [LB*j, LB*j+LB) of a W-word number, for every limb j */
+template <int W>
+CUFP_INL void words_to_limbs (u32 (&L)[limb_cut<W>::NL], const u32 (&A)[W])
+{
+  typedef limb_cut<W> C;
+  sfor<0, C::NL> ([&] (auto J) {
+    constexpr int j = decltype (J)::value;
+    constexpr int bit = C::LB * j, w = bit >> 5, s = bit & 31;
+    const u32 lo = A[w];
+    const u32 hi = (w + 1 < W) ? A[w + 1] : 0u;
+    L[j] = fshr (hi, lo, (u32) s) & C::MASK;
+  });
+}
+
+template <int W>
+CUFP_INL void mul_words_limbs (u32 (&p)[2 * W], const u32 (&a)[W], const u32 (&b)[W])
+{
+  typedef limb_cut<W> C;
+  constexpr int NL = C::NL, LB = C::LB;
+  u32 La[NL], Lb[NL], P[2 * NL];
+  words_to_limbs<W> (La, a);
+  words_to_limbs<W> (Lb, b);
+  u64 acc = 0;
+  sfor<0, 2 * NL - 1> ([&] (auto K) {
+    constexpr int k = decltype (K)::value;
+    constexpr int i0 = (k >= NL) ? k - NL + 1 : 0, i1 = (k < NL) ? k + 1 : NL;
+    sfor<i0, i1> ([&] (auto I) {
+      constexpr int i = decltype (I)::value;
+      acc = (u64) La[i] * Lb[k - i] + acc;
+    });
+    P[k] = (u32) acc & C::MASK;
+    acc >>= LB;
+  });
+  P[2 * NL - 1] = (u32) acc;
+  /* limbs -> 32-bit words: word q = bits [32q, 32q+32), spanning <= 3 limbs */
+  sfor<0, 2 * W> ([&] (auto Q) {
+    constexpr int q = decltype (Q)::value;
+    constexpr int k0 = (32 * q) / LB, s0 = 32 * q - LB * k0;
+    const u64 l0 = P[k0];
+    const u64 l1 = (k0 + 1 < 2 * NL) ? P[k0 + 1] : 0u;
+    const u64 l2 = (k0 + 2 < 2 * NL) ? P[k0 + 2] : 0u;
+    const u64 v = l0 | (l1 << LB) | (l2 << (2 * LB));
+    p[q] = (u32) (v >> s0);
+  });
+}
+
+template <int W>
+CUFP_INL void mul_words (u32 (&p)[2 * W], const u32 (&a)[W], const u32 (&b)[W])
+{
+  if constexpr (W >= 8)
+    mul_words_limbs<W> (p, a, b);
+  else
+    mul_words_rows<W> (p, a, b);
 }
 
 /* ---------------------------------------------------------------------------

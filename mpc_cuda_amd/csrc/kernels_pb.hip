@@ -82,23 +82,29 @@ __global__ __launch_bounds__ (TPB) void k_axpy_real (i64 n, const u64 *__restric
     }
 }
 
-/* z = a*x + y, MPC_RNDNN per operation */
+/* z = a*x + y, MPC_RNDNN per operation.  Two lanes per element: lane 2i+p
+ * computes part p (0 = Re, 1 = Im) of element i,
+ *   p = 0: RNDN(RNDN(ar*xr - ai*xi) + yr),   p = 1: RNDN(RNDN(ar*xi + ai*xr) + yi),
+ * with the operands selected so both lanes run the same instructions; each
+ * lane holds half the complex product's exact terms, which keeps the wide
+ * precisions in registers. */
 __global__ __launch_bounds__ (TPB) void k_axpy_cplx (i64 n, const u64 *__restrict__ al, const i64 *__restrict__ ase,
                                                      const u64 *__restrict__ xl, const i64 *__restrict__ xse,
                                                      const u64 *__restrict__ yl, const i64 *__restrict__ yse,
                                                      u64 *__restrict__ zl, i64 *__restrict__ zse)
 {
   const i64 o = (i64) NL * n;
-  const cu_fcomplex<PB> a (ldc<PB> (al, ase, 1, 0), ldc<PB> (al + NL, ase + 1, 1, 0));
-  i64 stride = (i64) gridDim.x * blockDim.x;
-  for (i64 i = (i64) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+  const cu_freal<PB> ar = ldc<PB> (al, ase, 1, 0), ai = ldc<PB> (al + NL, ase + 1, 1, 0);
+  const i64 stride = (i64) gridDim.x * blockDim.x;
+  for (i64 t = (i64) blockIdx.x * blockDim.x + threadIdx.x; t < 2 * n; t += stride)
     {
-      cu_fcomplex<PB> x (ld<PB> (xl, xse, n, i), ld<PB> (xl + o, xse + n, n, i));
-      cu_fcomplex<PB> t = cu_fp::cu_cmul<PB> (a, x);
-      cu_fcomplex<PB> y (ld<PB> (yl, yse, n, i), ld<PB> (yl + o, yse + n, n, i));
-      cu_fcomplex<PB> r = cu_fp::cu_cadd<PB> (t, y);
-      st<PB> (r.re, zl, zse, n, i);
-      st<PB> (r.im, zl + o, zse + n, n, i);
+      const i64 i = t >> 1;
+      const int p = (int) (t & 1);
+      const cu_freal<PB> xr = ldc<PB> (xl, xse, n, i), xi = ldc<PB> (xl + o, xse + n, n, i);
+      const cu_freal<PB> u = p ? xi : xr, v = p ? xr : xi;
+      cu_freal<PB> prod = cu_fp::core::fmm<PB> (ar, u, ai, v, p == 0);
+      cu_freal<PB> y = ld<PB> (yl + p * o, yse + p * n, n, i);
+      st<PB> (cu_fp::cu_fadd<PB> (prod, y), zl + p * o, zse + p * n, n, i);
     }
 }
 
@@ -237,7 +243,7 @@ int axpy_real (i64 n, const u64 *al, const i64 *ase, const u64 *xl, const i64 *x
 int axpy_cplx (i64 n, const u64 *al, const i64 *ase, const u64 *xl, const i64 *xse, const u64 *yl, const i64 *yse,
                u64 *ol, i64 *ose, hipStream_t s)
 {
-  hipLaunchKernelGGL (k_axpy_cplx, dim3 (grid_for (n, TPB, 32)), dim3 (TPB), 0, s, n, al, ase, xl, xse, yl, yse, ol, ose);
+  hipLaunchKernelGGL (k_axpy_cplx, dim3 (grid_for (2 * n, TPB, 32)), dim3 (TPB), 0, s, n, al, ase, xl, xse, yl, yse, ol, ose);
   return launch_status ("axpy_complex");
 }
 int gemm_perop (i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl, const i64 *Bse, u64 *Cl,
