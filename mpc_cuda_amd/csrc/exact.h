@@ -15,10 +15,10 @@
  *  3. MFMA     for an output tile, sum_k X_A X_B = sum_d 256^d G_d with
  *              G_d = sum_{t+u=d} sum_k DA[t][i][k] DB[u][j][k]; every (t,u)
  *              pair is an INT8 GEMM on v_mfma_i32_16x16x64_i8 accumulating
- *              exactly in int32 (|digit product| <= 2^14, <= G*K terms per
+ *              exactly in int32 (|digit product| <= 2^14, <= G*4096 terms per
  *              accumulator between flushes).  Digits are processed in groups of
- *              G so fragments and the 2G-1 diagonal accumulators of a pass fit
- *              in registers; each pass flushes its diagonals, shifted by 8d
+ *              G = 16: a pass keeps B's fragments in VGPRs, streams A's, and
+ *              holds the 2G-1 diagonal accumulators in AGPRs; each pass flushes its diagonals, shifted by 8d
  *              bits, into per-element carry-save int64 slots (32-bit steps).
  *  4. round    the slots are carry-normalised into a two's complement integer,
  *              negated if negative, and rounded once to pb bits (RNDN); the

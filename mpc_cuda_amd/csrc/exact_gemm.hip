@@ -61,7 +61,7 @@ size_t exact_workspace_bound (int pb, i64 m, i64 n, i64 k)
 {
   (void) pb;
   const i64 mp = pad_to (m, 16), np = pad_to (n, 16), kp = pad_to (k > 0 ? k : 1, 64);
-  const i64 spad = pad_to (EXACT_SMAX, 8);               /* nG*G for S <= EXACT_SMAX, G = 8 */
+  const i64 spad = pad_to (EXACT_SMAX, 16);              /* nG*G for S <= EXACT_SMAX, G = 16 */
   const i64 tiles = (mp / 16) * (np / 16);
   const i64 grid = tiles < 2048 ? tiles : 2048;
   size_t b = 0;
@@ -75,7 +75,7 @@ size_t exact_workspace_bound (int pb, i64 m, i64 n, i64 k)
 
 static void choose_groups (int S, int *G, int *nG)
 {
-  const int cand[1] = {8};
+  const int cand[1] = {16};
   long best = -1;
   for (int c : cand)
     {
@@ -134,6 +134,8 @@ int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, con
   if (!p.fallback)
     {
       choose_groups (p.S, &p.G, &p.nG);
+      /* fragment loads use 32-bit buffer offsets within one digit group */
+      if ((i64) p.G * (mp > np ? mp : np) * kp >= ((i64) 1 << 31)) return done (MPGFX_EUNSUPPORTED);
       const int spad = p.G * p.nG;
       p.slots = exact_slots (spad);
       const i64 tiles = (mp / 16) * (np / 16);

@@ -142,57 +142,98 @@ __device__ void round_store_int (i64 *__restrict__ wbuf, int nw, i64 scale, u64 
 
 /* The MFMA kernel: one wave per 16x16 output tile (persistent over tiles).
  * Lane l owns outputs (row 4*(l>>4)+v, col l&15), v < 4 (the 16x16 MFMA
- * accumulator layout); its carry-save slots live in `slotbuf`. */
+ * accumulator layout); its carry-save slots live in `slotbuf`.
+ *
+ * A pass pairs digit group ga of A with group gb of B (G digits each).  Per
+ * k step the wave holds the G fragments of B in VGPRs and streams the G
+ * fragments of A (the next one prefetched while the current one's G MFMAs
+ * run), so every 1 KB fragment load feeds G/2 MFMAs on average; the 2G-1
+ * diagonal accumulators stay in AGPRs for a whole k chunk.  Digit planes >= S
+ * of a partial last group are zero and their products are skipped with
+ * wave-uniform branches.  A pass's diagonals start at bit 8*G*(ga+gb), a
+ * multiple of 32, so they are folded in registers into PS slot-aligned 64-bit
+ * contributions and added to memory with all loads issued before the stores. */
 template <int G>
-__global__ __launch_bounds__ (64) void k_exact_mfma (i64 m, i64 n, i64 mp, i64 np, i64 kp, int nG, int slots,
+__global__ __launch_bounds__ (64, 2) void k_exact_mfma (i64 m, i64 n, i64 mp, i64 np, i64 kp, int nG, int S, int slots,
                                                      const int8_t *__restrict__ DA, const int8_t *__restrict__ DB,
                                                      const i64 *__restrict__ emaxA, const i64 *__restrict__ emaxB,
                                                      i64 sA, i64 sB, i64 *__restrict__ slotbuf,
                                                      u64 *__restrict__ Cl, i64 *__restrict__ Cse)
 {
+  static_assert ((8 * G) % 32 == 0, "pass offsets must be slot aligned");
+  constexpr int ND = 2 * G - 1;                    /* diagonals per pass */
+  constexpr int PS = (8 * (ND - 1) + 31) / 32 + 1;  /* slots a pass touches */
+  constexpr int KCH = EXACT_KCHUNK;                 /* G * KCH * 2^14 < 2^31 */
+  static_assert ((long long) G * KCH * 16384 < (1ll << 31), "int32 accumulators could overflow");
   const int l = threadIdx.x, r0 = l & 15, g = l >> 4;
   const i64 tn = np / 16, tiles = (mp / 16) * tn;
   i64 *my = slotbuf + ((i64) blockIdx.x * 64 + l) * 4 * slots;   /* 4 elements x slots */
+  const i64 pla = mp * kp, plb = np * kp;
   for (i64 tile = blockIdx.x; tile < tiles; tile += gridDim.x)
     {
       const i64 ti = tile / tn, tj = tile % tn;
       for (int s = 0; s < 4 * slots; s++) my[s] = 0;
-      const int8_t *pa = DA + (ti * 16 + r0) * kp + 16 * g;
-      const int8_t *pb = DB + (tj * 16 + r0) * kp + 16 * g;
-      const i64 pla = mp * kp, plb = np * kp;
+      /* fragment loads are raw buffer loads: one per-lane byte offset (row, k
+       * group) and a wave-uniform plane offset in an SGPR (t * plane size) */
+      const int voa = (int) ((ti * 16 + r0) * kp + 16 * g);
+      const int vob = (int) ((tj * 16 + r0) * kp + 16 * g);
       for (int ga = 0; ga < nG; ga++)
         for (int gb = 0; gb < nG; gb++)
-          for (i64 kc = 0; kc < kp; kc += EXACT_KCHUNK)
-            {
-              v4i acc[2 * G - 1];
+          {
+            const int tv = (S - ga * G < G) ? S - ga * G : G, uv = (S - gb * G < G) ? S - gb * G : G;
+            const bool full = (tv == G) && (uv == G);
+            const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc (
+              (void *) (DA + (i64) (ga * G) * pla), (short) 0, (int) (G * pla), 0x00020000);
+            const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc (
+              (void *) (DB + (i64) (gb * G) * plb), (short) 0, (int) (G * plb), 0x00020000);
+            for (i64 kc = 0; kc < kp; kc += KCH)
+              {
+                v4i acc[ND];
 #pragma unroll
-              for (int d = 0; d < 2 * G - 1; d++) acc[d] = (v4i) {0, 0, 0, 0};
-              const i64 kend = (kc + EXACT_KCHUNK < kp) ? kc + EXACT_KCHUNK : kp;
-              for (i64 k0 = kc; k0 < kend; k0 += 64)
-                {
-                  v4i fa[G], fb[G];
-#pragma unroll
-                  for (int t = 0; t < G; t++)
-                    fa[t] = *(const v4i *) (pa + (i64) (ga * G + t) * pla + k0);
-#pragma unroll
-                  for (int u = 0; u < G; u++)
-                    fb[u] = *(const v4i *) (pb + (i64) (gb * G + u) * plb + k0);
-#pragma unroll
-                  for (int t = 0; t < G; t++)
+                for (int d = 0; d < ND; d++) acc[d] = (v4i) {0, 0, 0, 0};
+                const i64 kend = (kc + KCH < kp) ? kc + KCH : kp;
+                for (i64 k0 = kc; k0 < kend; k0 += 64)
+                  {
+                    v4i fb[G];
 #pragma unroll
                     for (int u = 0; u < G; u++)
-                      acc[t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8 (fa[t], fb[u], acc[t + u], 0, 0, 0);
-                }
-              /* flush: diagonal (ga+gb)*G + d sits at bit 8*diag */
+                      fb[u] = __builtin_amdgcn_raw_buffer_load_b128 (rb, vob + (int) k0, (int) (u * plb), 0);
+                    v4i fa = __builtin_amdgcn_raw_buffer_load_b128 (ra, voa + (int) k0, 0, 0);
 #pragma unroll
-              for (int d = 0; d < 2 * G - 1; d++)
-                {
-                  const int bit = 8 * ((ga + gb) * G + d);
-                  const int s = bit >> 5, sh = bit & 31;
+                    for (int t = 0; t < G; t++)
+                      {
+                        if (t < tv)
+                          {
+                            const v4i fn = (t + 1 < tv)
+                              ? __builtin_amdgcn_raw_buffer_load_b128 (ra, voa + (int) k0, (int) ((t + 1) * pla), 0)
+                              : fa;
 #pragma unroll
-                  for (int v = 0; v < 4; v++) my[v * slots + s] += (i64) acc[d][v] << sh;
-                }
-            }
+                            for (int u = 0; u < G; u++)
+                              if (full || u < uv)
+                                acc[t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8 (fa, fb[u], acc[t + u], 0, 0, 0);
+                            fa = fn;
+                          }
+                      }
+                  }
+                /* fold the diagonals into PS slot-aligned contributions per element */
+                const int s0 = (8 * G * (ga + gb)) >> 5;
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                  {
+                    i64 c[PS];
+#pragma unroll
+                    for (int q = 0; q < PS; q++) c[q] = 0;
+#pragma unroll
+                    for (int d = 0; d < ND; d++) c[d >> 2] += (i64) acc[d][v] << (8 * (d & 3));
+                    i64 *sl = my + v * slots + s0;
+                    i64 old[PS];
+#pragma unroll
+                    for (int q = 0; q < PS; q++) old[q] = sl[q];
+#pragma unroll
+                    for (int q = 0; q < PS; q++) sl[q] = old[q] + c[q];
+                  }
+              }
+          }
       /* carry-normalise each element's slots into 32-bit words and round */
 #pragma unroll 1
       for (int v = 0; v < 4; v++)
@@ -292,10 +333,10 @@ int exact_mfma_launch (int G, const exact_plan &p, const int8_t *DA, const int8_
   if (G == GG)                                                                                             \
     {                                                                                                      \
       hipLaunchKernelGGL ((k_exact_mfma<GG>), dim3 (grid), dim3 (64), 0, s, p.m, p.n, p.mp, p.np, p.kp,    \
-                          p.nG, p.slots, DA, DB, emaxA, emaxB, p.sA, p.sB, slotbuf, Cl, Cse);              \
+                          p.nG, p.S, p.slots, DA, DB, emaxA, emaxB, p.sA, p.sB, slotbuf, Cl, Cse);         \
       return launch_status ("gemm_exact mfma");                                                            \
     }
-  EXACT_G (8)
+  EXACT_G (16)
 #undef EXACT_G
   return MPGFX_EINVAL;
 }
