@@ -2,48 +2,31 @@
  * batch.h).  Included by kernels_pb.hip inside its anonymous namespace (PB, NL
  * defined there). */
 
-/* value holder for the real / complex instantiation */
-template <int CPLX> struct bval;
-template <> struct bval<0>
-{
-  cu_freal<PB> v;
-  __device__ static bval load (const u64 *l, const i64 *s, i64 n, i64 i)
-  { bval r; r.v = ldc<PB> (l, s, n, i); return r; }
-  __device__ void store (u64 *l, i64 *s, i64 n, i64 i) const { st<PB> (v, l, s, n, i); }
-  __device__ static bval mul (const bval &a, const bval &b) { bval r; r.v = cu_fp::cu_fmul<PB> (a.v, b.v); return r; }
-  __device__ static bval add (const bval &a, const bval &b) { bval r; r.v = cu_fp::cu_fadd<PB> (a.v, b.v); return r; }
-  __device__ void zero () { v.set_zero (); }
-};
-template <> struct bval<1>
-{
-  cu_fcomplex<PB> v;
-  __device__ static bval load (const u64 *l, const i64 *s, i64 n, i64 i)
-  {
-    bval r;
-    r.v = cu_fcomplex<PB> (ldc<PB> (l, s, n, i), ldc<PB> (l + (i64) NL * n, s + n, n, i));
-    return r;
-  }
-  __device__ void store (u64 *l, i64 *s, i64 n, i64 i) const
-  {
-    st<PB> (v.re, l, s, n, i);
-    st<PB> (v.im, l + (i64) NL * n, s + n, n, i);
-  }
-  __device__ static bval mul (const bval &a, const bval &b) { bval r; r.v = cu_fp::cu_cmul<PB> (a.v, b.v); return r; }
-  __device__ static bval add (const bval &a, const bval &b) { bval r; r.v = cu_fp::cu_cadd<PB> (a.v, b.v); return r; }
-  __device__ void zero () { v.re.set_zero (); v.im.set_zero (); }
-};
-
 __device__ __forceinline__ i64 bt_scr_index (const bt_class &c, i64 p, i64 s)
 {
   return c.gbase[p >> 6] + 64 * s + (p & 63);
 }
 
-/* phase 1: workgroups pull items from a queue; lane per value */
+/* one part (0 real, 1 imaginary) of a product: for CPLX, the two lanes of an
+ * element run the same code, RNDN(ar*ur -/+ ai*vi) with the operands swapped
+ * and the operation chosen per lane (MPC's correctly rounded components) */
+template <int CPLX>
+__device__ __forceinline__ cu_freal<PB> bt_prod (const u64 *al, const i64 *as, i64 an, i64 ai_, const u64 *bl,
+                                                 const i64 *bs, i64 bn, i64 bi, int part)
+{
+  if (!CPLX) return cu_fp::cu_fmul<PB> (ldc<PB> (al, as, an, ai_), ldc<PB> (bl, bs, bn, bi));
+  const cu_freal<PB> ar = ldc<PB> (al, as, an, ai_), ai = ldc<PB> (al + (i64) NL * an, as + an, an, ai_);
+  const cu_freal<PB> br = ldc<PB> (bl, bs, bn, bi), bim = ldc<PB> (bl + (i64) NL * bn, bs + bn, bn, bi);
+  const cu_freal<PB> u = part ? bim : br, v = part ? br : bim;
+  return cu_fp::core::fmm<PB> (ar, u, ai, v, part == 0);
+}
+
+/* phase 1: workgroups pull items from a queue; lane per real value, two
+ * lanes (real and imaginary part) per complex value */
 template <int CPLX>
 __global__ __launch_bounds__ (256) void k_batch_products (bt_class c)
 {
   __shared__ unsigned slot;
-  typedef bval<CPLX> V;
   for (;;)
     {
       if (threadIdx.x == 0) slot = atomicAdd (&c.counters[0], 1u);
@@ -53,53 +36,64 @@ __global__ __launch_bounds__ (256) void k_batch_products (bt_class c)
       if (it >= c.nitems) break;
       const bt_item w = c.items[it];
       const bt_task &t = c.tasks[w.task];
-      for (i64 e = threadIdx.x; e < w.count; e += blockDim.x)
+      for (i64 e = threadIdx.x; e < (w.count << CPLX); e += blockDim.x)
         {
-          const i64 g = w.start + e;
+          const int part = CPLX ? (int) (e & 1) : 0;
+          const i64 g = w.start + (e >> CPLX);
           if (t.kind == MPGFX_TASK_AXPY)
             {
-              V a = V::load (t.in0l, t.in0s, 1, 0);
-              V r = V::add (V::mul (a, V::load (t.in1l, t.in1s, t.n, g)), V::load (t.in2l, t.in2s, t.n, g));
-              r.store (t.outl, t.outs, t.n, g);
-            }
-          else if (t.kind == MPGFX_TASK_DOT)
-            {
-              V p = V::mul (V::load (t.in0l, t.in0s, t.n, g), V::load (t.in1l, t.in1s, t.n, g));
-              p.store (c.scr_l, c.scr_s, c.scr_n, bt_scr_index (c, t.p0, g));
+              const cu_freal<PB> p = bt_prod<CPLX> (t.in0l, t.in0s, 1, 0, t.in1l, t.in1s, t.n, g, part);
+              const cu_freal<PB> y = ldc<PB> (t.in2l + (i64) part * NL * t.n, t.in2s + (i64) part * t.n, t.n, g);
+              st<PB> (cu_fp::cu_fadd<PB> (p, y), t.outl + (i64) part * NL * t.n, t.outs + (i64) part * t.n, t.n, g);
             }
           else
             {
-              const i64 i = g / t.n, k = g - i * t.n;
-              V p = V::mul (V::load (t.in0l, t.in0s, t.m * t.n, g), V::load (t.in1l, t.in1s, t.n, k));
-              p.store (c.scr_l, c.scr_s, c.scr_n, bt_scr_index (c, t.p0 + i, k));
+              i64 q;
+              cu_freal<PB> p;
+              if (t.kind == MPGFX_TASK_DOT)
+                {
+                  p = bt_prod<CPLX> (t.in0l, t.in0s, t.n, g, t.in1l, t.in1s, t.n, g, part);
+                  q = bt_scr_index (c, t.p0, g);
+                }
+              else
+                {
+                  const i64 i = g / t.n, k = g - i * t.n;
+                  p = bt_prod<CPLX> (t.in0l, t.in0s, t.m * t.n, g, t.in1l, t.in1s, t.n, k, part);
+                  q = bt_scr_index (c, t.p0 + i, k);
+                }
+              st<PB> (p, c.scr_l + (i64) part * NL * c.scr_n, c.scr_s + (i64) part * c.scr_n, c.scr_n, q);
             }
         }
     }
 }
 
-/* phase 2: a wavefront pulls a group of 64 chains; lane per chain */
+/* phase 2: a wavefront pulls 64 chain lanes: one real chain per lane, or, for
+ * complex classes, the real and the imaginary part of 32 chains (their folds
+ * are independent: complex addition is componentwise) */
 template <int CPLX>
 __global__ __launch_bounds__ (256) void k_batch_folds (bt_class c)
 {
-  typedef bval<CPLX> V;
   const int lane = threadIdx.x & 63;
+  const i64 units = CPLX ? 2 * c.ngroups : c.ngroups;
   for (;;)
     {
-      unsigned g = 0;
-      if (lane == 0) g = atomicAdd (&c.counters[1], 1u);
-      g = __shfl (g, 0);
-      if ((i64) g >= c.ngroups) break;
-      const bt_chain ch = c.chains[(i64) g * 64 + lane];
+      unsigned w = 0;
+      if (lane == 0) w = atomicAdd (&c.counters[1], 1u);
+      w = __shfl (w, 0);
+      if ((i64) w >= units) break;
+      const int part = CPLX ? (lane & 1) : 0;
+      const i64 p = CPLX ? (i64) w * 32 + (lane >> 1) : (i64) w * 64 + lane;   /* chain position */
+      const bt_chain ch = c.chains[p];
       if (ch.task >= 0)
         {
           const bt_task &t = c.tasks[ch.task];
-          V acc;
-          acc.zero ();
-          const i64 base = c.gbase[g] + lane;
-          for (i64 s = 0; s < ch.len; s++)
-            acc = V::add (acc, V::load (c.scr_l, c.scr_s, c.scr_n, base + 64 * s));
-          if (t.kind == MPGFX_TASK_DOT) acc.store (t.outl, t.outs, 1, 0);
-          else acc.store (t.outl, t.outs, t.m, ch.row);
+          const i64 base = c.gbase[p >> 6] + (p & 63);
+          const u64 *sl = c.scr_l + (i64) part * NL * c.scr_n;
+          const i64 *ss = c.scr_s + (i64) part * c.scr_n;
+          const cu_freal<PB> acc = fold_chain (sl, ss, c.scr_n, base, 64, ch.len);
+          const i64 nout = (t.kind == MPGFX_TASK_DOT) ? 1 : t.m;
+          const i64 idx = (t.kind == MPGFX_TASK_DOT) ? 0 : ch.row;
+          st<PB> (acc, t.outl + (i64) part * NL * nout, t.outs + (i64) part * nout, nout, idx);
         }
     }
 }

@@ -166,25 +166,71 @@ __global__ __launch_bounds__ (TPB) void k_gemm_cplx (i64 m, i64 n, i64 k, const 
     }
 }
 
-/* y[i] = fold_k RNDN(acc + RNDN(A[i,k]*x[k])): one lane per row; x[k] is a
- * broadcast load shared by the whole grid. */
-__global__ __launch_bounds__ (TPB) void k_gemv (i64 m, i64 k, const u64 *__restrict__ Al, const i64 *__restrict__ Ase,
-                                                const u64 *__restrict__ xl, const i64 *__restrict__ xse,
-                                                u64 *__restrict__ yl, i64 *__restrict__ yse)
+/* Sequential fold of one real chain: acc = +0; acc = RNDN(acc + p_s) for
+ * s < len, with the products of the next PD steps already loaded (a ring of
+ * registers rotated by a compile-time unrolled loop), so each step waits for
+ * the add only and not for memory. */
+constexpr int FOLD_PD = (NL >= 8) ? 2 : (NL >= 4 ? 4 : 8);
+
+__device__ __forceinline__ cu_freal<PB> fold_chain (const u64 *__restrict__ l, const i64 *__restrict__ se, i64 stride,
+                                                   i64 base, i64 step, i64 len)
 {
-  const i64 nA = m * k;
-  for (i64 i = (i64) blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (i64) gridDim.x * blockDim.x)
+  cu_freal<PB> acc;
+  acc.set_zero ();
+  cu_freal<PB> ring[FOLD_PD];
+#pragma unroll
+  for (int d = 0; d < FOLD_PD; d++)
+    if (d < len) ring[d] = ldc<PB> (l, se, stride, base + step * d);
+  for (i64 s0 = 0; s0 < len; s0 += FOLD_PD)
     {
-      cu_freal<PB> acc;
-      acc.set_zero ();
-      for (i64 kk = 0; kk < k; kk++)
+#pragma unroll
+      for (int d = 0; d < FOLD_PD; d++)
         {
-          cu_freal<PB> a = ldc<PB> (Al, Ase, nA, i * k + kk);
-          cu_freal<PB> x = ldc<PB> (xl, xse, k, kk);
-          acc = cu_fp::cu_fadd<PB> (acc, cu_fp::cu_fmul<PB> (a, x));
+          const i64 s = s0 + d;
+          if (s < len)
+            {
+              acc = cu_fp::cu_fadd<PB> (acc, ring[d]);
+              if (s + FOLD_PD < len) ring[d] = ldc<PB> (l, se, stride, base + step * (s + FOLD_PD));
+            }
         }
-      st<PB> (acc, yl, yse, m, i);
     }
+  return acc;
+}
+
+/* GEMV, per-op, in two phases.  Phase 1: every product P[i,k] = RNDN(A[i,k] *
+ * x[k]) in parallel, into a scratch MP array in row-group-interleaved order:
+ * rows in groups of 64, P[i,k] at (i/64)*64*K + k*64 + i%64, so that phase 2
+ * reads 64 consecutive values per step.  A wavefront computes one column k of
+ * one row group (x[k] is wave-uniform and sits in SGPRs).  Phase 2
+ * (fold_chain, shared with the batched execution): one lane per row, acc =
+ * RNDN(acc + P[i,k]) for k ascending, with the next products prefetched into
+ * registers. */
+__global__ __launch_bounds__ (TPB) void k_gemv_products (i64 m, i64 k, i64 r0, i64 mc, const u64 *__restrict__ Al,
+                                                         const i64 *__restrict__ Ase, const u64 *__restrict__ xl,
+                                                         const i64 *__restrict__ xse, u64 *__restrict__ Pl,
+                                                         i64 *__restrict__ Pse)
+{
+  /* rows r0 .. r0+mc-1 (mc a multiple of 64 or the last rows) */
+  const i64 nA = m * k, units = (mc + 63) / 64 * k, nP = units * 64;
+  const int lane = threadIdx.x & 63;
+  const i64 w0 = (i64) __builtin_amdgcn_readfirstlane ((int) (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)));
+  const i64 wstride = (i64) gridDim.x * (TPB / 64);
+  for (i64 u = w0; u < units; u += wstride)
+    {
+      const i64 g = u / k, kk = u - g * k, i = g * 64 + lane;
+      const cu_freal<PB> x = ldc<PB> (xl, xse, k, kk);
+      if (i < mc) st<PB> (cu_fp::cu_fmul<PB> (ld<PB> (Al, Ase, nA, (r0 + i) * k + kk), x), Pl, Pse, nP, u * 64 + lane);
+    }
+}
+
+__global__ __launch_bounds__ (64) void k_gemv_fold (i64 m, i64 k, i64 r0, i64 mc, const u64 *__restrict__ Pl,
+                                                    const i64 *__restrict__ Pse, u64 *__restrict__ yl,
+                                                    i64 *__restrict__ yse)
+{
+  const i64 i = (i64) blockIdx.x * 64 + threadIdx.x;
+  if (i >= mc) return;
+  const i64 nP = (mc + 63) / 64 * 64 * k;
+  st<PB> (fold_chain (Pl, Pse, nP, (i / 64) * 64 * k + (i & 63), 64, k), yl, yse, m, r0 + i);
 }
 
 /* AoS (reference struct layout) <-> SoA, one lane per real value */
@@ -262,13 +308,47 @@ int gemm_cplx (i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl
   hipLaunchKernelGGL (k_gemm_cplx, dim3 (g), dim3 (TPB), 0, s, m, n, k, Al, Ase, Bl, Bse, Cl, Cse);
   return launch_status ("gemm_complex");
 }
+/* product scratch budget of one GEMV call: 2 GiB, or MPGFX_GEMV_SCRATCH_BYTES */
+i64 gemv_scratch_budget ()
+{
+  const char *e = getenv ("MPGFX_GEMV_SCRATCH_BYTES");
+  const long long v = e ? atoll (e) : 0;
+  return v > 0 ? (i64) v : ((i64) 2 << 30);
+}
+
 int gemv (i64 m, i64 k, const u64 *Al, const i64 *Ase, const u64 *xl, const i64 *xse, u64 *yl, i64 *yse,
           hipStream_t s)
 {
-  /* 64-lane workgroups: one row per lane, as many workgroups as rows allow */
-  unsigned g = (unsigned) ((m + 63) / 64);
-  hipLaunchKernelGGL (k_gemv, dim3 (g), dim3 (64), 0, s, m, k, Al, Ase, xl, xse, yl, yse);
-  return launch_status ("gemv_real");
+  if (k == 0)
+    {
+      /* empty sums: +0 (the per-op GEMM kernel writes them) */
+      hipLaunchKernelGGL (k_gemm_perop, dim3 ((unsigned) (m < 65536 ? m : 65536)), dim3 (TPB), 0, s, m, (i64) 1,
+                          k, Al, Ase, xl, xse, yl, yse);
+      return launch_status ("gemv_real");
+    }
+  /* product scratch for as many 64-row groups as the budget holds (one at least) */
+  const i64 per_group = 64 * k * (NL + 1) * 8;
+  i64 groups = gemv_scratch_budget () / per_group;
+  if (groups < 1) groups = 1;
+  const i64 mc_max = (groups * 64 < m) ? groups * 64 : m;
+  const i64 cap = (mc_max + 63) / 64 * 64 * k;
+  u64 *Pl = nullptr;
+  hipError_t e = hipMallocAsync ((void **) &Pl, (size_t) cap * (NL + 1) * 8, s);
+  if (e != hipSuccess) { set_error ("gemv_real scratch", e); return MPGFX_ENOMEM; }
+  for (i64 r0 = 0; r0 < m; r0 += mc_max)
+    {
+      const i64 mc = (m - r0 < mc_max) ? m - r0 : mc_max;
+      const i64 units = (mc + 63) / 64 * k, nP = units * 64;
+      i64 *Pse = (i64 *) (Pl + (size_t) nP * NL);
+      const i64 blocks = (units + TPB / 64 - 1) / (TPB / 64);
+      hipLaunchKernelGGL (k_gemv_products, dim3 ((unsigned) (blocks < 65536 ? blocks : 65536)), dim3 (TPB), 0, s, m,
+                          k, r0, mc, Al, Ase, xl, xse, Pl, Pse);
+      hipLaunchKernelGGL (k_gemv_fold, dim3 ((unsigned) ((mc + 63) / 64)), dim3 (64), 0, s, m, k, r0, mc, Pl, Pse,
+                          yl, yse);
+    }
+  int rc = launch_status ("gemv_real");
+  (void) hipFreeAsync (Pl, s);
+  return rc;
 }
 int pack (int cplx, i64 n, const void *aos, u64 *l, i64 *se, hipStream_t s)
 {

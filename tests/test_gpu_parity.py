@@ -4,6 +4,7 @@ including the sign field of zeros, against the oracle (a restatement of the
 reference); zeros compared by value against MPFR fixtures (the reference's own
 tests do the same: tools/test_freal.cu:73)."""
 import numpy as np
+import torch
 import pytest
 
 import mpc_cuda_amd as M
@@ -108,6 +109,19 @@ def test_gemm_gemv_perop_vs_golden_and_oracle(gpu, pb):
     x = M.random_real(pb, kk, 4 + pb, -4, 4)
     r = M.gemv(A.cuda(), x.cuda(), mm, kk)
     assert same(r, O.gemm(pb, mm, 1, kk, A.pair(), x.pair(), "perop")) == 0
+
+
+def test_gemv_row_chunks(gpu, monkeypatch):
+    """GEMV with a product-scratch budget of one 64-row group: rows are
+    processed in chunks (200 rows = 3 full groups + 8), same results."""
+    pb, mm, kk = 128, 200, 33
+    A = M.random_real(pb, mm * kk, 51, -4, 4)
+    x = M.random_real(pb, kk, 52, -4, 4)
+    ref = O.gemm(pb, mm, 1, kk, A.pair(), x.pair(), "perop")
+    monkeypatch.setenv("MPGFX_GEMV_SCRATCH_BYTES", "1")
+    r = M.gemv(A.cuda(), x.cuda(), mm, kk)
+    torch.cuda.synchronize()
+    assert same(r, ref) == 0
 
 
 def test_empty_and_single_element(gpu):
