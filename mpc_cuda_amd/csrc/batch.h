@@ -6,10 +6,11 @@
  *           or products of a dot / GEMV, each written (rounded, as the per-op
  *           fold needs) into the product scratch;
  *  phase 2  `chains`: one per dot and per GEMV row, sorted by length
- *           (longest first) and grouped by 64; a wavefront folds one group, one
- *           chain per lane.  Step s of the chain at position p lives at scratch
- *           index gbase[p/64] + 64*s + p%64, so a wavefront's loads at step s
- *           are contiguous.
+ *           (longest first) and grouped by R = 64 / coop_lanes(pb); a
+ *           wavefront folds one group, coop_lanes(pb) lanes per chain.  The
+ *           product scratch holds records of N limbs + the sign/exponent word;
+ *           step s of the chain at position p is record gbase[p/R] + R*s + p%R,
+ *           so a wavefront's loads at step s are contiguous.
  */
 #ifndef MPGFX_BATCH_H
 #define MPGFX_BATCH_H
@@ -49,9 +50,8 @@ struct bt_class
   const bt_chain *chains;
   const i64 *gbase;
   i64 ngroups;
-  u64 *scr_l;                               /* product scratch MP array */
-  i64 *scr_s;
-  i64 scr_n;                                /* values (per part when complex) */
+  u64 *scr;                                  /* product records (re part, then im part) */
+  i64 scr_n;                                /* records per part */
   unsigned *counters;                       /* [0] phase-1 queue, [1] phase-2 queue */
 };
 

@@ -349,24 +349,27 @@ extern "C" int mpgfx_batch_create (const mpgfx_task *tasks, int64_t ntasks, int 
       std::stable_sort (ch.begin (), ch.end (), [] (const bt_chain &a, const bt_chain &b2) { return a.len > b2.len; });
       for (int64_t p = (int64_t) ch.size () - 1; p >= 0; p--) c.tasks[ch[p].task].p0 = p;   /* first position */
       c.nchains = (int64_t) ch.size ();
-      int64_t ngroups = (c.nchains + 63) / 64;
+      /* fold groups of R chains: a wavefront folds R chains at once, G = 64/R
+       * lanes per number */
+      const int64_t R = 64 / coop_lanes (c.pb);
+      int64_t ngroups = (c.nchains + R - 1) / R;
       std::vector<int64_t> gbase (ngroups > 0 ? ngroups : 1, 0);
       int64_t acc = 0;
       for (int64_t g = 0; g < ngroups; g++)
         {
           gbase[g] = acc;
-          acc += 64 * ch[g * 64].len;                     /* longest chain of the group */
-          c.total_chain_steps += ch[g * 64].len;
+          acc += R * ch[g * R].len;                       /* longest chain of the group */
+          c.total_chain_steps += ch[g * R].len;
         }
-      while ((int64_t) ch.size () < ngroups * 64) ch.push_back ({-1, 0, 0});
-      const int64_t scr_n = acc;
+      while ((int64_t) ch.size () < ngroups * R) ch.push_back ({-1, 0, 0});
+      const int64_t scr_n = acc;                          /* records per part */
       const int nl = (c.pb + 63) / 64, parts = c.cplx ? 2 : 1;
       /* product items: larger tasks first */
       std::stable_sort (c.axpy_like.begin (), c.axpy_like.end (),
                         [] (const std::pair<int, int64_t> &a, const std::pair<int, int64_t> &b2) { return a.second > b2.second; });
       double mean_task = c.axpy_like.empty () ? 0.0 : (double) c.total_values / (double) c.axpy_like.size ();
       c.bucket1 = bucket_of (mean_task);
-      c.bucket2 = bucket_of (c.nchains ? (double) c.total_chain_steps * 64.0 / (double) c.nchains : 0.0);
+      c.bucket2 = bucket_of (c.nchains ? (double) c.total_chain_steps * (double) R / (double) c.nchains : 0.0);
       /* device allocations */
       size_t tb = c.tasks.size () * sizeof (bt_task);
       if (!rc) rc = hip_ok ("batch tasks", hipMalloc (&c.d_tasks, tb));
@@ -379,18 +382,16 @@ extern "C" int mpgfx_batch_create (const mpgfx_task *tasks, int64_t ntasks, int 
           if (!rc) rc = hip_ok ("batch gbase", hipMalloc (&c.d_gbase, gbase.size () * sizeof (int64_t)));
           if (!rc) rc = hip_ok ("batch gbase copy", hipMemcpy (c.d_gbase, gbase.data (), gbase.size () * sizeof (int64_t),
                                                                hipMemcpyHostToDevice));
-          size_t lb = (size_t) scr_n * nl * parts * 8, sb = (size_t) scr_n * parts * 8;
+          const size_t lb = (size_t) scr_n * (nl + 1) * parts * 8;   /* records of nl limbs + sign/exponent */
           if (!rc) rc = hip_ok ("batch scratch", hipMalloc (&c.d_scr_l, lb ? lb : 8));
-          if (!rc) rc = hip_ok ("batch scratch", hipMalloc (&c.d_scr_s, sb ? sb : 8));
-          b->scratch_bytes += lb + sb;
+          b->scratch_bytes += lb;
         }
       if (!rc) rc = hip_ok ("batch counters", hipMalloc (&c.d_counters, 64));
       c.dev.tasks = (const bt_task *) c.d_tasks;
       c.dev.chains = (const bt_chain *) c.d_chains;
       c.dev.gbase = (const i64 *) c.d_gbase;
       c.dev.ngroups = ngroups;
-      c.dev.scr_l = (u64 *) c.d_scr_l;
-      c.dev.scr_s = (i64 *) c.d_scr_s;
+      c.dev.scr = (u64 *) c.d_scr_l;
       c.dev.scr_n = scr_n;
       c.dev.counters = (unsigned *) c.d_counters;
       if (!rc) rc = upload_items (c, c.map1.chunk, s);

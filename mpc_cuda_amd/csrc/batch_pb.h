@@ -2,9 +2,10 @@
  * batch.h).  Included by kernels_pb.hip inside its anonymous namespace (PB, NL
  * defined there). */
 
+/* record of step s of the chain at position p (records of NL+1 words) */
 __device__ __forceinline__ i64 bt_scr_index (const bt_class &c, i64 p, i64 s)
 {
-  return c.gbase[p >> 6] + 64 * s + (p & 63);
+  return c.gbase[p / COOP_R] + COOP_R * s + p % COOP_R;
 }
 
 /* one part (0 real, 1 imaginary) of a product: for CPLX, the two lanes of an
@@ -61,39 +62,40 @@ __global__ __launch_bounds__ (256) void k_batch_products (bt_class c)
                   p = bt_prod<CPLX> (t.in0l, t.in0s, t.m * t.n, g, t.in1l, t.in1s, t.n, k, part);
                   q = bt_scr_index (c, t.p0 + i, k);
                 }
-              st<PB> (p, c.scr_l + (i64) part * NL * c.scr_n, c.scr_s + (i64) part * c.scr_n, c.scr_n, q);
+              st_rec (p, c.scr + ((i64) part * c.scr_n + q) * (NL + 1));
             }
         }
     }
 }
 
-/* phase 2: a wavefront pulls 64 chain lanes: one real chain per lane, or, for
- * complex classes, the real and the imaginary part of 32 chains (their folds
- * are independent: complex addition is componentwise) */
+/* phase 2: a wavefront pulls a fold group (R chains, or for complex classes
+ * the real or the imaginary parts of R chains: complex addition is
+ * componentwise, so the two parts fold independently) and folds it
+ * cooperatively, COOP_G lanes per chain */
 template <int CPLX>
 __global__ __launch_bounds__ (256) void k_batch_folds (bt_class c)
 {
   const int lane = threadIdx.x & 63;
-  const i64 units = CPLX ? 2 * c.ngroups : c.ngroups;
+  const i64 units = c.ngroups << CPLX;
   for (;;)
     {
       unsigned w = 0;
       if (lane == 0) w = atomicAdd (&c.counters[1], 1u);
       w = __shfl (w, 0);
       if ((i64) w >= units) break;
-      const int part = CPLX ? (lane & 1) : 0;
-      const i64 p = CPLX ? (i64) w * 32 + (lane >> 1) : (i64) w * 64 + lane;   /* chain position */
-      const bt_chain ch = c.chains[p];
+      const i64 fg = (i64) w >> CPLX;
+      const int part = CPLX ? (int) (w & 1) : 0;
+      const int r = lane / COOP_G;
+      const bt_chain ch = c.chains[fg * COOP_R + r];
+      const i64 max_len = c.chains[fg * COOP_R].len;            /* longest first */
+      const i64 len = ch.task >= 0 ? ch.len : 0;
+      const cval acc = coop_fold (c.scr + (i64) part * c.scr_n * (NL + 1), c.gbase[fg] + r, COOP_R, len, max_len);
       if (ch.task >= 0)
         {
           const bt_task &t = c.tasks[ch.task];
-          const i64 base = c.gbase[p >> 6] + (p & 63);
-          const u64 *sl = c.scr_l + (i64) part * NL * c.scr_n;
-          const i64 *ss = c.scr_s + (i64) part * c.scr_n;
-          const cu_freal<PB> acc = fold_chain (sl, ss, c.scr_n, base, 64, ch.len);
           const i64 nout = (t.kind == MPGFX_TASK_DOT) ? 1 : t.m;
           const i64 idx = (t.kind == MPGFX_TASK_DOT) ? 0 : ch.row;
-          st<PB> (acc, t.outl + (i64) part * NL * nout, t.outs + (i64) part * nout, nout, idx);
+          coop_store (acc, t.outl + (i64) part * NL * nout, t.outs + (i64) part * nout, nout, idx);
         }
     }
 }

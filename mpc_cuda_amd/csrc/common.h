@@ -62,6 +62,14 @@ int launch_status (const char *where);
 
 inline int nlimbs (int pb) { return (pb + 63) / 64; }
 
+/* lanes per number of the cooperative folds: the power of two >= N + 2 (a
+ * guard limb, N limbs, a carry limb), at least 4 */
+__host__ __device__ constexpr int coop_lanes (int pb)
+{
+  return ((pb + 63) / 64 + 2 <= 4) ? 4 : ((pb + 63) / 64 + 2 <= 8) ? 8 : ((pb + 63) / 64 + 2 <= 16) ? 16
+       : ((pb + 63) / 64 + 2 <= 32) ? 32 : 64;
+}
+
 } // namespace mpgfx
 
 /* precisions compiled into the elementwise / AXPY / GEMV / GEMM kernels */

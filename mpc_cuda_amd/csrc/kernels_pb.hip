@@ -166,71 +166,38 @@ __global__ __launch_bounds__ (TPB) void k_gemm_cplx (i64 m, i64 n, i64 k, const 
     }
 }
 
-/* Sequential fold of one real chain: acc = +0; acc = RNDN(acc + p_s) for
- * s < len, with the products of the next PD steps already loaded (a ring of
- * registers rotated by a compile-time unrolled loop), so each step waits for
- * the add only and not for memory. */
-constexpr int FOLD_PD = (NL >= 8) ? 2 : (NL >= 4 ? 4 : 8);
-
-__device__ __forceinline__ cu_freal<PB> fold_chain (const u64 *__restrict__ l, const i64 *__restrict__ se, i64 stride,
-                                                   i64 base, i64 step, i64 len)
-{
-  cu_freal<PB> acc;
-  acc.set_zero ();
-  cu_freal<PB> ring[FOLD_PD];
-#pragma unroll
-  for (int d = 0; d < FOLD_PD; d++)
-    if (d < len) ring[d] = ldc<PB> (l, se, stride, base + step * d);
-  for (i64 s0 = 0; s0 < len; s0 += FOLD_PD)
-    {
-#pragma unroll
-      for (int d = 0; d < FOLD_PD; d++)
-        {
-          const i64 s = s0 + d;
-          if (s < len)
-            {
-              acc = cu_fp::cu_fadd<PB> (acc, ring[d]);
-              if (s + FOLD_PD < len) ring[d] = ldc<PB> (l, se, stride, base + step * (s + FOLD_PD));
-            }
-        }
-    }
-  return acc;
-}
+#include "coop.h"
 
 /* GEMV, per-op, in two phases.  Phase 1: every product P[i,k] = RNDN(A[i,k] *
- * x[k]) in parallel, into a scratch MP array in row-group-interleaved order:
- * rows in groups of 64, P[i,k] at (i/64)*64*K + k*64 + i%64, so that phase 2
- * reads 64 consecutive values per step.  A wavefront computes one column k of
- * one row group (x[k] is wave-uniform and sits in SGPRs).  Phase 2
- * (fold_chain, shared with the batched execution): one lane per row, acc =
- * RNDN(acc + P[i,k]) for k ascending, with the next products prefetched into
- * registers. */
+ * x[k]) in parallel (lane per product), into records of N limbs + the
+ * sign/exponent word, ordered for the fold: rows in fold groups of R =
+ * 64/COOP_G, record (i/R)*K*R + k*R + i%R, so that a fold step of a group is R
+ * consecutive records.  Phase 2: a wavefront per fold group, COOP_G lanes per
+ * row, acc = RNDN(acc + P[i,k]) for k ascending (coop_fold). */
 __global__ __launch_bounds__ (TPB) void k_gemv_products (i64 m, i64 k, i64 r0, i64 mc, const u64 *__restrict__ Al,
                                                          const i64 *__restrict__ Ase, const u64 *__restrict__ xl,
-                                                         const i64 *__restrict__ xse, u64 *__restrict__ Pl,
-                                                         i64 *__restrict__ Pse)
+                                                         const i64 *__restrict__ xse, u64 *__restrict__ P)
 {
-  /* rows r0 .. r0+mc-1 (mc a multiple of 64 or the last rows) */
-  const i64 nA = m * k, units = (mc + 63) / 64 * k, nP = units * 64;
-  const int lane = threadIdx.x & 63;
-  const i64 w0 = (i64) __builtin_amdgcn_readfirstlane ((int) (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)));
-  const i64 wstride = (i64) gridDim.x * (TPB / 64);
-  for (i64 u = w0; u < units; u += wstride)
+  /* rows r0 .. r0+mc-1 */
+  const i64 nA = m * k, kr = k * COOP_R, nrec = (mc + COOP_R - 1) / COOP_R * kr;
+  const i64 stride = (i64) gridDim.x * blockDim.x;
+  for (i64 e = (i64) blockIdx.x * blockDim.x + threadIdx.x; e < nrec; e += stride)
     {
-      const i64 g = u / k, kk = u - g * k, i = g * 64 + lane;
-      const cu_freal<PB> x = ldc<PB> (xl, xse, k, kk);
-      if (i < mc) st<PB> (cu_fp::cu_fmul<PB> (ld<PB> (Al, Ase, nA, (r0 + i) * k + kk), x), Pl, Pse, nP, u * 64 + lane);
+      const i64 fg = e / kr, rem = e - fg * kr, kk = rem / COOP_R, i = fg * COOP_R + (rem - kk * COOP_R);
+      if (i < mc)
+        st_rec (cu_fp::cu_fmul<PB> (ld<PB> (Al, Ase, nA, (r0 + i) * k + kk), ldc<PB> (xl, xse, k, kk)),
+                P + e * (NL + 1));
     }
 }
 
-__global__ __launch_bounds__ (64) void k_gemv_fold (i64 m, i64 k, i64 r0, i64 mc, const u64 *__restrict__ Pl,
-                                                    const i64 *__restrict__ Pse, u64 *__restrict__ yl,
-                                                    i64 *__restrict__ yse)
+__global__ __launch_bounds__ (64) void k_gemv_fold (i64 m, i64 k, i64 r0, i64 mc, const u64 *__restrict__ P,
+                                                    u64 *__restrict__ yl, i64 *__restrict__ yse)
 {
-  const i64 i = (i64) blockIdx.x * 64 + threadIdx.x;
-  if (i >= mc) return;
-  const i64 nP = (mc + 63) / 64 * 64 * k;
-  st<PB> (fold_chain (Pl, Pse, nP, (i / 64) * 64 * k + (i & 63), 64, k), yl, yse, m, r0 + i);
+  const i64 fg = blockIdx.x;
+  const int r = (int) (threadIdx.x / COOP_G);
+  const i64 i = fg * COOP_R + r;
+  const cval acc = coop_fold (P, fg * k * COOP_R + r, COOP_R, i < mc ? k : 0, k);
+  if (i < mc) coop_store (acc, yl, yse, m, r0 + i);
 }
 
 /* AoS (reference struct layout) <-> SoA, one lane per real value */
@@ -326,28 +293,25 @@ int gemv (i64 m, i64 k, const u64 *Al, const i64 *Ase, const u64 *xl, const i64 
                           k, Al, Ase, xl, xse, yl, yse);
       return launch_status ("gemv_real");
     }
-  /* product scratch for as many 64-row groups as the budget holds (one at least) */
-  const i64 per_group = 64 * k * (NL + 1) * 8;
+  /* product scratch for as many fold groups as the budget holds (one at least) */
+  const i64 per_group = COOP_R * k * (NL + 1) * 8;
   i64 groups = gemv_scratch_budget () / per_group;
   if (groups < 1) groups = 1;
-  const i64 mc_max = (groups * 64 < m) ? groups * 64 : m;
-  const i64 cap = (mc_max + 63) / 64 * 64 * k;
-  u64 *Pl = nullptr;
-  hipError_t e = hipMallocAsync ((void **) &Pl, (size_t) cap * (NL + 1) * 8, s);
+  const i64 mc_max = (groups * COOP_R < m) ? groups * COOP_R : m;
+  const i64 cap = (mc_max + COOP_R - 1) / COOP_R * COOP_R * k;
+  u64 *P = nullptr;
+  hipError_t e = hipMallocAsync ((void **) &P, (size_t) cap * (NL + 1) * 8, s);
   if (e != hipSuccess) { set_error ("gemv_real scratch", e); return MPGFX_ENOMEM; }
   for (i64 r0 = 0; r0 < m; r0 += mc_max)
     {
       const i64 mc = (m - r0 < mc_max) ? m - r0 : mc_max;
-      const i64 units = (mc + 63) / 64 * k, nP = units * 64;
-      i64 *Pse = (i64 *) (Pl + (size_t) nP * NL);
-      const i64 blocks = (units + TPB / 64 - 1) / (TPB / 64);
-      hipLaunchKernelGGL (k_gemv_products, dim3 ((unsigned) (blocks < 65536 ? blocks : 65536)), dim3 (TPB), 0, s, m,
-                          k, r0, mc, Al, Ase, xl, xse, Pl, Pse);
-      hipLaunchKernelGGL (k_gemv_fold, dim3 ((unsigned) ((mc + 63) / 64)), dim3 (64), 0, s, m, k, r0, mc, Pl, Pse,
-                          yl, yse);
+      const i64 nfg = (mc + COOP_R - 1) / COOP_R;
+      hipLaunchKernelGGL (k_gemv_products, dim3 (grid_for (nfg * COOP_R * k, TPB, 32)), dim3 (TPB), 0, s, m, k, r0,
+                          mc, Al, Ase, xl, xse, P);
+      hipLaunchKernelGGL (k_gemv_fold, dim3 ((unsigned) nfg), dim3 (64), 0, s, m, k, r0, mc, P, yl, yse);
     }
   int rc = launch_status ("gemv_real");
-  (void) hipFreeAsync (Pl, s);
+  (void) hipFreeAsync (P, s);
   return rc;
 }
 int pack (int cplx, i64 n, const void *aos, u64 *l, i64 *se, hipStream_t s)

@@ -111,6 +111,35 @@ def test_gemm_gemv_perop_vs_golden_and_oracle(gpu, pb):
     assert same(r, O.gemm(pb, mm, 1, kk, A.pair(), x.pair(), "perop")) == 0
 
 
+@pytest.mark.parametrize("pb", PBS)
+def test_fold_edge_cases(gpu, pb):
+    """The GEMV fold adds RNDN(RNDN(+0 + a*1) + b*1) = RNDN(a + b) for every
+    structured edge pair (ties, carry-out, cancellation to +0 and to one bit,
+    far operands with sticky borrow, zero operands): bit-exact against the
+    oracle's k-ordered fold, whose additions are pinned against MPFR."""
+    ea, eb, _, _ = edge_arrays(pb, seed=29)
+    rows = ea.n
+    A = MPArray(pb, np.ascontiguousarray(np.stack([ea.limbs, eb.limbs], 2).reshape(ea.limbs.shape[0], 2 * rows)),
+                np.ascontiguousarray(np.stack([ea.se, eb.se], 1).reshape(2 * rows)), False)
+    N = M.nlimbs(pb)
+    one_l = np.zeros((N, 2), np.uint64)
+    one_l[N - 1, :] = np.uint64(1 << 63)
+    x = MPArray(pb, one_l, np.array([2, 2], np.int64), False)      # 1.0, 1.0
+    r = M.gemv(A.cuda(), x.cuda(), rows, 2)
+    assert same(r, O.gemm(pb, rows, 1, 2, A.pair(), x.pair(), "perop")) == 0
+
+
+@pytest.mark.parametrize("pb", PBS)
+def test_gemv_wide_exponents(gpu, pb):
+    """Per-op GEMV with exponents in [-150, 150]: far alignments across
+    several limbs, sticky-only additions, and cancellations in the fold."""
+    mm, kk = 130, 37
+    A = M.random_real(pb, mm * kk, 71 + pb, -150, 150)
+    x = M.random_real(pb, kk, 72 + pb, -150, 150)
+    r = M.gemv(A.cuda(), x.cuda(), mm, kk)
+    assert same(r, O.gemm(pb, mm, 1, kk, A.pair(), x.pair(), "perop")) == 0
+
+
 def test_gemv_row_chunks(gpu, monkeypatch):
     """GEMV with a product-scratch budget of one 64-row group: rows are
     processed in chunks (200 rows = 3 full groups + 8), same results."""
