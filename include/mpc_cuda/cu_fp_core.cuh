@@ -85,6 +85,17 @@ CUFP_INL u32 fshl (u32 hi, u32 lo, u32 s)
 #endif
 }
 
+/* true if the condition holds in any active lane of the wavefront (a
+ * wave-uniform branch condition: steps no lane needs are skipped by all) */
+CUFP_INL bool any_lane (bool c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_ballot_w64 (c) != 0;
+#else
+  return c;
+#endif
+}
+
 CUFP_INL int clz32 (u32 x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -295,6 +306,7 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
     {
       const int sh = 1 << k;
       u32 on = (ws >> k) & 1u;
+      if (!any_lane (on != 0)) continue;        /* step needed by no lane */
       u32 lost = 0;
       CUFP_UNROLL
       for (int i = 0; i < sh && i < L; i++) lost |= T[i];
@@ -358,6 +370,7 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
         {
           const int sh = 1 << k;
           u32 on = (wl >> k) & 1u;
+          if (!any_lane (on != 0)) continue;
           CUFP_UNROLL
           for (int i = L - 2; i >= 0; i--)
             {
