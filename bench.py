@@ -39,6 +39,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--extra", action="store_true", help="also time the other configurations")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--traffic-driver", action="store_true",
+                    help="only run 3 headline launches and 3 calibration copies (for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -49,6 +51,9 @@ def main():
     import mpc_cuda_amd as M
 
     torch.cuda.set_device(0)
+    if args.traffic_driver:
+        traffic_driver()
+        return
     a = M.random_complex(PB, 1, 201, -16, 16).cuda()
     x = M.random_complex(PB, N_ELEM, 202, -16, 16).cuda()
     y = M.random_complex(PB, N_ELEM, 203, -16, 16).cuda()
@@ -89,7 +94,7 @@ def main():
         "config": {"workload": "complex AXPY z=a*x+y, cu_fcomplex<1024>, n=2^18, MPC_RNDNN per op",
                    "pb": PB, "n": N_ELEM, "mp_ops_per_element": 2},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_axpy_cplx<1024>", "kernel_ms": kern_ms,
                      "algorithmic_bytes_per_launch": BYTES_PER_ELEM * N_ELEM,
                      "word_products_per_s": wp_rate},
@@ -104,6 +109,38 @@ def main():
         rec["extra"] = extra_configs(args)
 
     print(json.dumps(rec))
+
+
+def traffic_driver():
+    """The launches profiled for profiles/axpy_cplx_pb1024_traffic.md: three
+    headline AXPYs on device-generated inputs, then three 512 MiB device
+    copies whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
+    import torch
+    import mpc_cuda_amd as M
+    a = M.device_random(PB, 1, 1, -16, 16, cplx=True)
+    x = M.device_random(PB, N_ELEM, 2, -16, 16, cplx=True)
+    y = M.device_random(PB, N_ELEM, 3, -16, 16, cplx=True)
+    for _ in range(3):
+        M.axpy(a, x, y)
+    src = torch.empty(1 << 29, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print("traffic driver done", flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per launch of the headline kernel at this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (calibrated against a
+    copy of known size): profiles/axpy_cplx_pb1024_traffic.md."""
+    import re
+    try:
+        with open(os.path.join(ROOT, "profiles", "axpy_cplx_pb1024_traffic.md")) as f:
+            m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", f.read(), re.M)
+        return int(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def cpu_baseline(a, x, y, z):

@@ -17,8 +17,9 @@
  *              pair is an INT8 GEMM on v_mfma_i32_16x16x64_i8 accumulating
  *              exactly in int32 (|digit product| <= 2^14, <= G*4096 terms per
  *              accumulator between flushes).  Digits are processed in groups of
- *              G = 16: a pass keeps B's fragments in VGPRs, streams A's, and
- *              holds the 2G-1 diagonal accumulators in AGPRs; each pass flushes its diagonals, shifted by 8d
+ *              G = 16 or 20 (fewest passes): a pass keeps B's fragments in
+ *              VGPRs, streams A's, and holds the 2G-1 diagonal accumulators in
+ *              AGPRs; each pass flushes its diagonals, shifted by 8d
  *              bits, into per-element carry-save int64 slots (32-bit steps).
  *  4. round    the slots are carry-normalised into a two's complement integer,
  *              negated if negative, and rounded once to pb bits (RNDN); the
@@ -35,12 +36,15 @@ namespace mpgfx {
 
 constexpr int EXACT_SMAX = 64;          /* digits per operand on the MFMA path */
 constexpr i64 EXACT_EMPTY = INT64_MIN;  /* row/column without nonzero entries */
-constexpr int EXACT_KCHUNK = 4096;      /* int32 flush period in k (G <= 16) */
+constexpr int EXACT_KCHUNK = 4096;      /* int32 flush period in k (G <= 20) */
 
 inline int exact_digits (int pb, i64 spread) { return (int) ((pb + spread + 2 + 7) / 8); }
 inline i64 pad_to (i64 x, i64 m) { return (x + m - 1) / m * m; }
 /* carry-save slots per output element for S digits (diagonals up to 2S-2) */
 inline int exact_slots (int S) { return (8 * (2 * S - 1) + 31) / 32 + 2; }
+/* slots for the largest padded digit count (pad_to(EXACT_SMAX, 16) digits) */
+constexpr int EXACT_SLMAX = (8 * (2 * 64 - 1) + 31) / 32 + 2;
+static_assert (EXACT_SMAX <= 64, "EXACT_SLMAX assumes at most 64 padded digits");
 
 /* per-launch plan, filled on the host after the scan */
 struct exact_plan

@@ -56,32 +56,42 @@ static int hip_ok (const char *where, hipError_t e)
   return MPGFX_OK;
 }
 
+/* workgroups of the MFMA kernel: one per CU (persistent over 32x32 tiles), a
+ * multiple of 8 so the XCD-aware tile assignment covers every tile */
+static i64 exact_grid (i64 tiles)
+{
+  const i64 g = tiles < 256 ? tiles : 256;
+  return (g + 7) / 8 * 8;
+}
+
 /* bytes of workspace for the MFMA path with digit count up to EXACT_SMAX */
 size_t exact_workspace_bound (int pb, i64 m, i64 n, i64 k)
 {
   (void) pb;
-  const i64 mp = pad_to (m, 16), np = pad_to (n, 16), kp = pad_to (k > 0 ? k : 1, 64);
+  const i64 mp = pad_to (m, 32), np = pad_to (n, 32), kp = pad_to (k > 0 ? k : 1, 64);
   const i64 spad = pad_to (EXACT_SMAX, 16);              /* nG*G for S <= EXACT_SMAX, G = 16 */
-  const i64 tiles = (mp / 16) * (np / 16);
-  const i64 grid = tiles < 2048 ? tiles : 2048;
+  const i64 grid = exact_grid ((mp / 32) * (np / 32));
   size_t b = 0;
   b += (size_t) (2 * m + 2 * n + 8) * 8;                 /* scan outputs + spread words */
   b = (size_t) pad_to ((i64) b, 256);
   b += (size_t) (spad * (mp + np) * kp);                 /* digit planes */
   b = (size_t) pad_to ((i64) b, 256);
-  b += (size_t) grid * 64 * 4 * exact_slots ((int) spad) * 8;
+  b += (size_t) grid * 256 * 4 * exact_slots ((int) spad) * 8;
   return b;
 }
 
+/* digits per group: fewer passes first (a pass re-stages its planes for every
+ * k step, so passes with few digits are load-latency bound), then the smaller
+ * group; the padded digit count G*nG stays <= 64 (workspace bound) */
 static void choose_groups (int S, int *G, int *nG)
 {
-  const int cand[1] = {16};
-  long best = -1;
+  const int cand[2] = {16, 20};
+  int best_passes = -1;
   for (int c : cand)
     {
-      int ng = (S + c - 1) / c;
-      long work = (long) (ng * c) * (ng * c);            /* padded digit pairs */
-      if (best < 0 || work < best) { best = work; *G = c; *nG = ng; }
+      const int ng = (S + c - 1) / c;
+      if (ng * c > 64) continue;
+      if (best_passes < 0 || ng * ng < best_passes) { best_passes = ng * ng; *G = c; *nG = ng; }
     }
 }
 
@@ -90,7 +100,7 @@ int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, con
 {
   if (k == 0)                      /* empty sums: +0 everywhere, as the per-op path */
     return o->gemm_perop (m, n, k, Al, Ase, Bl, Bse, Cl, Cse, s);
-  const i64 mp = pad_to (m, 16), np = pad_to (n, 16), kp = pad_to (k, 64);
+  const i64 mp = pad_to (m, 32), np = pad_to (n, 32), kp = pad_to (k, 64);
   /* ---- scan (always needs a small buffer) ---- */
   size_t scan_bytes = (size_t) pad_to ((i64) ((2 * m + 2 * n + 8) * 8), 256);
   char *base = (char *) ws;
@@ -138,8 +148,7 @@ int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, con
       if ((i64) p.G * (mp > np ? mp : np) * kp >= ((i64) 1 << 31)) return done (MPGFX_EUNSUPPORTED);
       const int spad = p.G * p.nG;
       p.slots = exact_slots (spad);
-      const i64 tiles = (mp / 16) * (np / 16);
-      const int grid = (int) (tiles < 2048 ? tiles : 2048);
+      const int grid = (int) exact_grid ((mp / 32) * (np / 32));
       char *q = base + scan_bytes;
       int8_t *DA = (int8_t *) q;
       q += pad_to ((i64) spad * mp * kp, 256);

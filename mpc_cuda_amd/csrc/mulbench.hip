@@ -86,6 +86,28 @@ __global__ __launch_bounds__ (256) void k_generic (int pairs, int reps, int w, c
   out[i] = x;
 }
 
+/* INT8 MFMA peak: 8 independent v_mfma_i32_16x16x64_i8 accumulator chains
+ * per wavefront (16*16*64*2 = 32768 ops each) */
+typedef int v4i __attribute__ ((ext_vector_type (4)));
+__global__ __launch_bounds__ (256) void k_mfma_peak (int iters, int seed, int *out)
+{
+  v4i a = {seed + (int) threadIdx.x, seed ^ 0x1234567, seed * 3, seed - 7};
+  v4i b = {seed ^ (int) blockIdx.x, seed + 11, seed * 5, seed ^ 0x7654321};
+  v4i acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) acc[j] = (v4i) {j, 0, 0, 0};
+  for (int it = 0; it < iters; it++)
+    {
+#pragma unroll
+      for (int j = 0; j < 8; j++) acc[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8 (a, b, acc[j], 0, 0, 0);
+      a.x ^= it;
+    }
+  int x = 0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) x ^= acc[j].x ^ acc[j].y ^ acc[j].z ^ acc[j].w;
+  out[(size_t) blockIdx.x * blockDim.x + threadIdx.x] = x;
+}
+
 int main (int argc, char **argv)
 {
   int pairs = argc > 1 ? atoi (argv[1]) : (1 << 16);
@@ -113,6 +135,23 @@ int main (int argc, char **argv)
     printf ("{\"bench\": \"mad_u64_u32_peak\", \"cus\": %d, \"clock_ghz\": %.3f, \"ms\": %.4f, "
             "\"word_products_per_s\": %.4e, \"per_cu_per_clk\": %.3f}\n",
             cus, clk_ghz, ms, mads / (ms * 1e-3), mads / (ms * 1e-3) / cus / (clk_ghz * 1e9));
+    HIPCHK (hipFree (out));
+  }
+  /* ---- INT8 MFMA peak ---- */
+  {
+    int blocks = cus * 8, iters = 1 << 12;
+    int *out;
+    HIPCHK (hipMalloc (&out, (size_t) blocks * 256 * 4));
+    hipLaunchKernelGGL (k_mfma_peak, dim3 (blocks), dim3 (256), 0, 0, 16, 5, out);
+    HIPCHK (hipEventRecord (e0));
+    hipLaunchKernelGGL (k_mfma_peak, dim3 (blocks), dim3 (256), 0, 0, iters, 5, out);
+    HIPCHK (hipEventRecord (e1));
+    HIPCHK (hipEventSynchronize (e1));
+    HIPCHK (hipEventElapsedTime (&ms, e0, e1));
+    double ops = (double) blocks * 4 * iters * 8 * 32768.0;
+    printf ("{\"bench\": \"mfma_i32_16x16x64_i8_peak\", \"ms\": %.4f, \"ops_per_s\": %.4e, "
+            "\"cycles_per_mfma_per_simd\": %.2f}\n",
+            ms, ops / (ms * 1e-3), (ms * 1e-3) * clk_ghz * 1e9 * cus * 4 / (ops / 32768.0));
     HIPCHK (hipFree (out));
   }
   /* ---- 1024-bit products: register-resident vs generic ---- */

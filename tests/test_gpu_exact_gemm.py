@@ -35,6 +35,8 @@ def test_exact_gemm_vs_golden(gpu, pb):
     (512, 20, 24, 129, -4, 4),
     (96, 18, 19, 70, -40, 40),      # sub-limb precision (SB = 32)
     (1024, 16, 16, 65, -16, 16),
+    (64, 70, 40, 4200, -4, 4),      # k beyond one int32 accumulation chunk (4096)
+    (64, 544, 512, 64, -4, 4),      # more 32x32 tiles (272) than workgroups (256)
 ])
 def test_exact_gemm_vs_oracle(gpu, pb, m, n, k, lo, hi):
     A = M.random_real(pb, m * k, 11 + pb + k, lo, hi)
