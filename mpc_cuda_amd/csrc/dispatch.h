@@ -27,8 +27,8 @@ struct pb_ops
   /* exact-dot GEMM stages (exact_pb.h) */
   int (*exact_digits) (i64 lines, i64 len, i64 ls, i64 ps, const u64 *l, const i64 *se, i64 nv, const i64 *emax,
                        i64 spread, int S, int Spad, i64 Lpad, i64 kp, int8_t *D, hipStream_t);
-  int (*exact_mfma) (int G, const exact_plan &p, const int8_t *DA, const int8_t *DB, const i64 *emaxA,
-                     const i64 *emaxB, i64 *slotbuf, int grid, u64 *Cl, i64 *Cse, hipStream_t);
+  int (*exact_round) (i64 m, i64 n, i64 np, i64 nE, int slots, const i64 *S64, const i64 *emaxA, const i64 *emaxB,
+                      i64 sA, i64 sB, u64 *Cl, i64 *Cse, hipStream_t);
   int (*exact_fallback) (i64 m, i64 n, i64 k, i64 r0, i64 rows, const u64 *Al, const i64 *Ase, const u64 *Bl,
                          const i64 *Bse, const i64 *eminA, const i64 *eminB, int nw, i64 *wsp, u64 *Cl, i64 *Cse,
                          hipStream_t);

@@ -50,18 +50,221 @@ __global__ __launch_bounds__ (256) void k_exact_scan (i64 lines, i64 len, i64 ls
     }
 }
 
+
+typedef int v4i __attribute__ ((ext_vector_type (4)));
+typedef __attribute__ ((address_space (3))) void *lds_ptr;
+
+/* ---------------------------------------------------------------------------
+ * The MFMA kernel (precision-independent: it sees only digit planes).
+ *
+ * A workgroup of 4 wavefronts computes a 32x32 output tile (wave w: rows
+ * 16*(w>>1).., columns 16*(w&1)..); lane l of a wave owns the outputs (row
+ * 4*(l>>4)+v, column l&15), v < 4 (the 16x16 MFMA accumulator layout).
+ *
+ * A pass pairs digit group ga of A with group gb of B (G digits each; planes
+ * past S hold zero digits, so every pass is a full G x G pass and the MFMA
+ * stream has no branches).  Per k step of 64 the G planes of both groups for
+ * the tile's 32 rows and 32 columns are copied into LDS with direct
+ * global->LDS loads (64 lanes x 16 bytes land at consecutive LDS addresses; the
+ * lane->global mapping applies an XOR swizzle, chunk c of row r at position
+ * c ^ ((r>>1)&3), so the 16-byte fragment reads of 8 consecutive rows hit
+ * distinct banks).  Two workgroups share a CU: one computes while the other
+ * waits for its loads.  A wave reads the G fragments of B into VGPRs and
+ * streams the G fragments of A: acc[t+u] += A_t B_u on v_mfma_i32_16x16x64_i8
+ * (2G-1 diagonal int32 accumulators, exact for G * EXACT_KCHUNK * 2^14 < 2^31).
+ * Diagonal d of a pass sits at digit G*(ga+gb)+d, i.e. slot (G*(ga+gb)+d)/4
+ * shifted by 8*((G*(ga+gb)+d) mod 4) bits; each pass adds its diagonals into
+ * the element's carry-save int64 slots, slot-major S64[q*nE + i*np + j] (the
+ * first pass of a tile stores them and zeroes the rest).
+ *
+ * Tiles are assigned XCD-aware: workgroup b runs on XCD b % 8, and the
+ * workgroups of one XCD take consecutive tiles (rows of tiles), so the A rows
+ * they share stay in that XCD's L2. */
+template <int G>
+struct exact_cfg
+{
+  static constexpr int ND = 2 * G - 1;                 /* diagonals per pass */
+  static constexpr int PS = ((3 + ND - 1) >> 2) + 1;   /* slots a pass touches (digit offset up to 3) */
+  static constexpr int PLANE = 32 * 64;                /* LDS bytes of one plane of the tile */
+  static constexpr int LDS = 2 * G * PLANE;            /* A planes, then B planes */
+};
+
+/* add the diagonals of one pass into the element's slots, digit offset O */
+template <int G, int O>
+__device__ __forceinline__ void exact_fold (const v4i (&acc)[exact_cfg<G>::ND], int v, i64 (&c)[exact_cfg<G>::PS])
+{
+#pragma unroll
+  for (int q = 0; q < exact_cfg<G>::PS; q++) c[q] = 0;
+#pragma unroll
+  for (int d = 0; d < exact_cfg<G>::ND; d++) c[(O + d) >> 2] += (i64) acc[d][v] << (8 * ((O + d) & 3));
+}
+
+template <int G>
+__global__ __launch_bounds__ (256, 2) void k_exact_mfma (i64 mp, i64 np, i64 kp, int nG, int slots,
+                                                       const int8_t *__restrict__ DA, const int8_t *__restrict__ DB,
+                                                       i64 *__restrict__ S64)
+{
+  typedef exact_cfg<G> C;
+  constexpr int ND = C::ND, PS = C::PS, PLANE = C::PLANE;
+  constexpr int KCH = EXACT_KCHUNK;
+  static_assert ((long long) G * KCH * 16384 < (1ll << 31), "int32 accumulators could overflow");
+  static_assert (4 * G % 4 == 0 && G <= 20, "group size");
+  __shared__ __attribute__ ((aligned (16))) int8_t lds[C::LDS];
+
+  const int tid = threadIdx.x, l = tid & 63, w = __builtin_amdgcn_readfirstlane (tid >> 6), wr = w >> 1, wc = w & 1;
+  const int r0 = l & 15, g = l >> 4;
+  const i64 tn = np / 32, tiles = (mp / 32) * tn, nE = mp * np;
+  const i64 pla = mp * kp, plb = np * kp;
+  const i64 nb = gridDim.x;
+  const i64 per_xcd = nb / 8, xslot = (i64) (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  /* staging: the wave copies G blocks of 1 KB (16 rows x 64 bytes) per step;
+   * block b = w*G + x is half b&1 of plane b/2 of A (b < 2G), or of B.  Lane l
+   * lands at row 16*half + l/4, position l&3, and fetches chunk
+   * (l&3) ^ ((l/4 >> 1) & 3) of that row: one per-lane offset for all blocks */
+  const u32 lane_off = (u32) ((l >> 2) * kp + 16 * ((l & 3) ^ (((l >> 2) >> 1) & 3)));
+  /* fragment reads: row r of the tile plane, 16-byte chunk g, swizzled */
+  const int ra = wr * 16 + r0, rb = wc * 16 + r0;
+  const int offa = ra * 64 + 16 * (g ^ ((ra >> 1) & 3));
+  const int offb = G * PLANE + rb * 64 + 16 * (g ^ ((rb >> 1) & 3));
+
+  for (i64 it = 0; it * nb < tiles; it++)
+    {
+      const i64 tile = it * nb + xslot;
+      if (tile >= tiles) break;
+      const i64 ti = tile / tn, tj = tile % tn;
+      for (int ga = 0; ga < nG; ga++)
+        for (int gb = 0; gb < nG; gb++)
+          {
+            const bool first = (ga == 0) && (gb == 0);
+            /* buffer resources over this pass's digit groups (G planes each);
+             * 32-bit offsets: G * max(mp, np) * kp < 2^31 is checked on the host */
+            const __amdgpu_buffer_rsrc_t rsa = __builtin_amdgcn_make_buffer_rsrc (
+              (void *) (DA + (i64) (ga * G) * pla), (short) 0, (int) (G * pla), 0x00020000);
+            const __amdgpu_buffer_rsrc_t rsb = __builtin_amdgcn_make_buffer_rsrc (
+              (void *) (DB + (i64) (gb * G) * plb), (short) 0, (int) (G * plb), 0x00020000);
+            for (i64 kc = 0; kc < kp; kc += KCH)
+              {
+                const i64 kend = (kc + KCH < kp) ? kc + KCH : kp;
+                v4i acc[ND];
+#pragma unroll
+                for (int d = 0; d < ND; d++) acc[d] = (v4i) {0, 0, 0, 0};
+                for (i64 k0 = kc; k0 < kend; k0 += 64)
+                  {
+                    /* stage the planes of this step */
+#pragma unroll
+                    for (int x = 0; x < G; x++)
+                      {
+                        const int b = w * G + x;                  /* wave-uniform */
+                        const bool isB = b >= 2 * G;
+                        const int pb = isB ? b - 2 * G : b, t = pb >> 1, half = pb & 1;
+                        const u32 so = (u32) (isB ? t * plb + (tj * 32 + 16 * half) * kp + k0
+                                                  : t * pla + (ti * 32 + 16 * half) * kp + k0);
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds (
+                          isB ? rsb : rsa, (lds_ptr) (lds + (isB ? G * PLANE : 0) + t * PLANE + half * 1024), 16,
+                          lane_off, so, 0, 0);
+                      }
+                    __builtin_amdgcn_s_waitcnt (0);
+                    __syncthreads ();
+                    v4i fb[G];
+#pragma unroll
+                    for (int u = 0; u < G; u++) fb[u] = *(const v4i *) (lds + offb + u * PLANE);
+                    v4i fa = *(const v4i *) (lds + offa);
+#pragma unroll
+                    for (int t = 0; t < G; t++)
+                      {
+                        const v4i fn = (t + 1 < G) ? *(const v4i *) (lds + offa + (t + 1) * PLANE) : fa;
+#pragma unroll
+                        for (int u = 0; u < G; u++)
+                          acc[t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8 (fa, fb[u], acc[t + u], 0, 0, 0);
+                        fa = fn;
+                      }
+                    __syncthreads ();
+                  }
+                /* add the diagonals into the slots of the 4 elements; the
+                 * opaque copy of nE keeps the slot addresses from being
+                 * precomputed (and held in registers) outside the k loop */
+                const int base = G * (ga + gb);
+                const int s0 = base >> 2, o = base & 3;
+                i64 nEo = nE;
+                asm volatile ("" : "+s"(nEo));
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                  {
+                    const i64 el = (ti * 32 + wr * 16 + 4 * g + v) * np + tj * 32 + wc * 16 + r0;
+                    i64 cs[PS];
+                    if (G % 4 == 0 || o == 0) exact_fold<G, 0> (acc, v, cs);
+                    else if (o == 1) exact_fold<G, (G % 4 == 0) ? 0 : 1> (acc, v, cs);
+                    else if (o == 2) exact_fold<G, (G % 4 == 0) ? 0 : 2> (acc, v, cs);
+                    else exact_fold<G, (G % 4 == 0) ? 0 : 3> (acc, v, cs);
+                    i64 *sl = S64 + (i64) s0 * nEo + el;
+                    if (first && kc == 0)
+                      {
+#pragma unroll
+                        for (int q = 0; q < PS; q++) sl[(i64) q * nEo] = cs[q];
+                        for (int q = s0 + PS; q < slots; q++) S64[(i64) q * nEo + el] = 0;
+                      }
+                    else
+                      {
+                        i64 old[PS];
+#pragma unroll
+                        for (int q = 0; q < PS; q++) old[q] = (s0 + q < slots) ? sl[(i64) q * nEo] : 0;
+#pragma unroll
+                        for (int q = 0; q < PS; q++)
+                          if (s0 + q < slots) sl[(i64) q * nEo] = old[q] + cs[q];
+                      }
+                  }
+              }
+          }
+    }
+}
+
+int exact_mfma_run (const exact_plan &p, const int8_t *DA, const int8_t *DB, i64 *S64, int grid, hipStream_t s)
+{
+#define EXACT_G(GG)                                                                                         \
+  if (p.G == GG)                                                                                            \
+    {                                                                                                       \
+      hipLaunchKernelGGL ((k_exact_mfma<GG>), dim3 (grid), dim3 (256), 0, s, p.mp, p.np, p.kp, p.nG, p.slots, \
+                          DA, DB, S64);                                                                     \
+      return launch_status ("gemm_exact mfma");                                                             \
+    }
+  EXACT_G (8)
+  EXACT_G (12)
+  EXACT_G (16)
+  EXACT_G (17)
+#undef EXACT_G
+  return MPGFX_EINVAL;
+}
+
 static int hip_ok (const char *where, hipError_t e)
 {
   if (e != hipSuccess) { set_error (where, e); return MPGFX_EHIP; }
   return MPGFX_OK;
 }
 
-/* workgroups of the MFMA kernel: one per CU (persistent over 32x32 tiles), a
+/* workgroups of the MFMA kernel: two per CU (persistent over 32x32 tiles), a
  * multiple of 8 so the XCD-aware tile assignment covers every tile */
 static i64 exact_grid (i64 tiles)
 {
-  const i64 g = tiles < 256 ? tiles : 256;
+  const i64 g = tiles < 512 ? tiles : 512;
   return (g + 7) / 8 * 8;
+}
+
+/* digit groups: G in {8, 12, 16, 17} and nG = ceil(S/G) with the padded digit
+ * count G*nG <= 64 (zero planes past S); least padded work (G*nG)^2 first,
+ * then fewest passes */
+static void choose_groups (int S, int *G, int *nG)
+{
+  const int cand[4] = {8, 12, 16, 17};
+  long best = -1;
+  int best_passes = 0;
+  for (int c : cand)
+    {
+      const int ng = (S + c - 1) / c;
+      if (ng * c > 64) continue;
+      const long work = (long) (ng * c) * (ng * c);
+      if (best < 0 || work < best || (work == best && ng * ng < best_passes))
+        { best = work; best_passes = ng * ng; *G = c; *nG = ng; }
+    }
 }
 
 /* bytes of workspace for the MFMA path with digit count up to EXACT_SMAX */
@@ -69,30 +272,14 @@ size_t exact_workspace_bound (int pb, i64 m, i64 n, i64 k)
 {
   (void) pb;
   const i64 mp = pad_to (m, 32), np = pad_to (n, 32), kp = pad_to (k > 0 ? k : 1, 64);
-  const i64 spad = pad_to (EXACT_SMAX, 16);              /* nG*G for S <= EXACT_SMAX, G = 16 */
-  const i64 grid = exact_grid ((mp / 32) * (np / 32));
+  const i64 spad = 64;                                   /* G*nG <= 64 */
   size_t b = 0;
   b += (size_t) (2 * m + 2 * n + 8) * 8;                 /* scan outputs + spread words */
   b = (size_t) pad_to ((i64) b, 256);
   b += (size_t) (spad * (mp + np) * kp);                 /* digit planes */
   b = (size_t) pad_to ((i64) b, 256);
-  b += (size_t) grid * 256 * 4 * exact_slots ((int) spad) * 8;
+  b += (size_t) mp * np * EXACT_SLMAX * 8;               /* carry-save slots */
   return b;
-}
-
-/* digits per group: fewer passes first (a pass re-stages its planes for every
- * k step, so passes with few digits are load-latency bound), then the smaller
- * group; the padded digit count G*nG stays <= 64 (workspace bound) */
-static void choose_groups (int S, int *G, int *nG)
-{
-  const int cand[2] = {16, 20};
-  int best_passes = -1;
-  for (int c : cand)
-    {
-      const int ng = (S + c - 1) / c;
-      if (ng * c > 64) continue;
-      if (best_passes < 0 || ng * ng < best_passes) { best_passes = ng * ng; *G = c; *nG = ng; }
-    }
 }
 
 int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl,
@@ -144,7 +331,7 @@ int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, con
   if (!p.fallback)
     {
       choose_groups (p.S, &p.G, &p.nG);
-      /* fragment loads use 32-bit buffer offsets within one digit group */
+      /* the MFMA kernel addresses one digit group with 32-bit offsets */
       if ((i64) p.G * (mp > np ? mp : np) * kp >= ((i64) 1 << 31)) return done (MPGFX_EUNSUPPORTED);
       const int spad = p.G * p.nG;
       p.slots = exact_slots (spad);
@@ -154,10 +341,11 @@ int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, con
       q += pad_to ((i64) spad * mp * kp, 256);
       int8_t *DB = (int8_t *) q;
       q += pad_to ((i64) spad * np * kp, 256);
-      i64 *slotbuf = (i64 *) q;
+      i64 *S64 = (i64 *) q;
       if ((rc = o->exact_digits (m, k, k, 1, Al, Ase, m * k, emaxA, p.sA, p.S, spad, mp, kp, DA, s))) return done (rc);
       if ((rc = o->exact_digits (n, k, 1, n, Bl, Bse, k * n, emaxB, p.sB, p.S, spad, np, kp, DB, s))) return done (rc);
-      rc = o->exact_mfma (p.G, p, DA, DB, emaxA, emaxB, slotbuf, grid, Cl, Cse, s);
+      if ((rc = exact_mfma_run (p, DA, DB, S64, grid, s))) return done (rc);
+      rc = o->exact_round (m, n, np, mp * np, p.slots, S64, emaxA, emaxB, p.sA, p.sB, Cl, Cse, s);
       return done (rc);
     }
   /* ---- VALU wide-accumulator fallback, in row chunks ---- */

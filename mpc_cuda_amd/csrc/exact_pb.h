@@ -141,180 +141,34 @@ __device__ void round_store_int (W *__restrict__ wbuf, int nw, i64 scale, u64 *_
   st<PB> (r, Cl, Cse, nC, i);
 }
 
-/* The MFMA kernel.  A workgroup of 4 wavefronts computes a 32x32 output tile
- * (wave w: rows 16*(w>>1).., columns 16*(w&1)..); lane l of a wave owns the
- * outputs (row 4*(l>>4)+v, column l&15), v < 4 (the 16x16 MFMA accumulator
- * layout), with its carry-save slots in `slotbuf`.
- *
- * A pass pairs digit group ga of A with digit group gb of B (G digits each).
- * Per k step of 64 the workgroup stages the G planes of both groups for its 32
- * rows and 32 columns in LDS (64 KB, each byte fetched from memory once per
- * workgroup and read by two waves); the next step's planes are loaded into
- * registers while the MFMAs of the current one run.  LDS rows are padded to
- * 80 bytes so the 16-byte fragment reads of 8 consecutive rows hit distinct
- * banks.  A wave reads the G fragments of B into VGPRs and streams the G
- * fragments of A: acc[t+u] += A_t B_u on v_mfma_i32_16x16x64_i8 (2G-1
- * diagonal int32 accumulators, exact for G * EXACT_KCHUNK * 2^14 < 2^31).
- * Digits of a partial last group are zero; their loads and products are
- * skipped.  Each pass's diagonals start at bit 8*G*(ga+gb), a multiple of 32,
- * and are folded in registers into PS slot-aligned 64-bit contributions.
- *
- * Tiles are assigned XCD-aware: workgroup b runs on XCD b % 8, and the 32
- * workgroups of one XCD take 32 consecutive tiles (one row of tiles at 1024
- * columns), so the A rows they share stay in that XCD's L2. */
-template <int G>
-__global__ __launch_bounds__ (256, 1) void k_exact_lds (i64 m, i64 n, i64 mp, i64 np, i64 kp, int nG, int S, int slots,
-                                                      const int8_t *__restrict__ DA, const int8_t *__restrict__ DB,
-                                                      const i64 *__restrict__ emaxA, const i64 *__restrict__ emaxB,
-                                                      i64 sA, i64 sB, i64 *__restrict__ slotbuf,
-                                                      u64 *__restrict__ Cl, i64 *__restrict__ Cse)
+/* Rounding stage: one lane per output element.  The MFMA kernel leaves, for
+ * element (i, j), `slots` carry-save int64 slots (slot q weighs 2^(32q), slot-
+ * major: S64[q * nE + i*np + j]); they are carry-normalised into 32-bit
+ * two's complement words, staged in LDS and rounded once to pb bits. */
+__global__ __launch_bounds__ (256) void k_exact_round (i64 m, i64 n, i64 np, i64 nE, int slots,
+                                                       const i64 *__restrict__ S64, const i64 *__restrict__ emaxA,
+                                                       const i64 *__restrict__ emaxB, i64 sA, i64 sB,
+                                                       u64 *__restrict__ Cl, i64 *__restrict__ Cse)
 {
-  static_assert (G % 2 == 0, "staging assignment splits the planes over two thread halves");
-  static_assert ((8 * G) % 32 == 0, "pass offsets must be slot aligned");
-  constexpr int ND = 2 * G - 1;                    /* diagonals per pass */
-  constexpr int PS = (8 * (ND - 1) + 31) / 32 + 1;  /* slots a pass touches */
-  constexpr int KCH = EXACT_KCHUNK;
-  static_assert ((long long) G * KCH * 16384 < (1ll << 31), "int32 accumulators could overflow");
-  constexpr int RS = 80;                           /* LDS row stride (bytes) */
-  constexpr int PL = 32 * RS;                      /* LDS plane: 32 rows */
-  constexpr int NST = G;                           /* 16-byte chunks staged per thread per step */
-  constexpr int HG = G / 2;
-  __shared__ __attribute__ ((aligned (16))) int8_t lds[2 * G * PL];
-  __shared__ u32 words[256 * EXACT_SLMAX];         /* epilogue: one lane's integer */
-
-  const int tid = threadIdx.x, l = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
-  const int r0 = l & 15, g = l >> 4;
-  /* staging: chunk x of this thread is plane 2*(x % (G/2)) + tid/128 of A
-   * (x < G/2) or of B (x >= G/2), row (tid/4) & 31, 16-byte column tid & 3 */
-  const int srow = (tid >> 2) & 31, scol = tid & 3, shalf = tid >> 7;
-  const i64 tn = np / 32, tiles = (mp / 32) * tn;
-  const i64 pla = mp * kp, plb = np * kp;
-  const i64 nb = gridDim.x;
-  const i64 per_xcd = nb / 8, xslot = (i64) (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-  i64 *my = slotbuf + ((i64) blockIdx.x * 256 + tid) * 4 * slots;
-  for (i64 it = 0; it * nb < tiles; it++)
+  __shared__ u32 words[256 * EXACT_SLMAX];
+  const i64 e = (i64) blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m * n) return;
+  const i64 i = e / n, j = e - i * n, el = i * np + j;
+  i64 x[EXACT_SLMAX];
+#pragma unroll
+  for (int q = 0; q < EXACT_SLMAX; q++) x[q] = (q < slots) ? S64[(i64) q * nE + el] : 0;
+  u32 *wb = words + threadIdx.x * EXACT_SLMAX;
+  i64 carry = 0;
+#pragma unroll
+  for (int q = 0; q < EXACT_SLMAX; q++)
     {
-      const i64 tile = it * nb + xslot;
-      if (tile >= tiles) break;
-      const i64 ti = tile / tn, tj = tile % tn;
-      for (int q = 0; q < 4 * slots; q++) my[q] = 0;
-      for (int ga = 0; ga < nG; ga++)
-        for (int gb = 0; gb < nG; gb++)
-          {
-            const int tv = (S - ga * G < G) ? S - ga * G : G, uv = (S - gb * G < G) ? S - gb * G : G;
-            const bool full = (tv == G) && (uv == G);
-            const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc (
-              (void *) (DA + (i64) (ga * G) * pla), (short) 0, (int) (G * pla), 0x00020000);
-            const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc (
-              (void *) (DB + (i64) (gb * G) * plb), (short) 0, (int) (G * plb), 0x00020000);
-            const int goa = (int) ((ti * 32 + srow) * kp + 16 * scol);
-            const int gob = (int) ((tj * 32 + srow) * kp + 16 * scol);
-            for (i64 kc = 0; kc < kp; kc += KCH)
-              {
-                const i64 kend = (kc + KCH < kp) ? kc + KCH : kp;
-                v4i stg[NST];
-                auto fetch = [&] (i64 k0) {
-#pragma unroll
-                  for (int x = 0; x < NST; x++)
-                    {
-                      const int t = 2 * (x % HG) + shalf;
-                      const bool isA = x < HG;
-                      const bool need = isA ? (t < tv) : (t < uv);
-                      stg[x] = need ? (isA ? __builtin_amdgcn_raw_buffer_load_b128 (ra, goa + (int) k0, (int) (t * pla), 0)
-                                           : __builtin_amdgcn_raw_buffer_load_b128 (rb, gob + (int) k0, (int) (t * plb), 0))
-                                    : (v4i) {0, 0, 0, 0};
-                    }
-                };
-                auto stash = [&] () {
-#pragma unroll
-                  for (int x = 0; x < NST; x++)
-                    {
-                      const int t = 2 * (x % HG) + shalf;
-                      *(v4i *) (lds + (x < HG ? 0 : G * PL) + t * PL + srow * RS + 16 * scol) = stg[x];
-                    }
-                };
-                v4i acc[ND];
-#pragma unroll
-                for (int d = 0; d < ND; d++) acc[d] = (v4i) {0, 0, 0, 0};
-                fetch (kc);
-                stash ();
-                __syncthreads ();
-                for (i64 k0 = kc; k0 < kend; k0 += 64)
-                  {
-                    const bool more = k0 + 64 < kend;
-                    if (more) fetch (k0 + 64);
-                    v4i fb[G];
-#pragma unroll
-                    for (int u = 0; u < G; u++)
-                      fb[u] = *(const v4i *) (lds + G * PL + u * PL + (wc * 16 + r0) * RS + 16 * g);
-                    const int8_t *pa = lds + (wr * 16 + r0) * RS + 16 * g;
-                    v4i fa = *(const v4i *) pa;
-#pragma unroll
-                    for (int t = 0; t < G; t++)
-                      {
-                        if (t < tv)
-                          {
-                            const v4i fn = (t + 1 < tv) ? *(const v4i *) (pa + (t + 1) * PL) : fa;
-#pragma unroll
-                            for (int u = 0; u < G; u++)
-                              if (full || u < uv)
-                                acc[t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8 (fa, fb[u], acc[t + u], 0, 0, 0);
-                            fa = fn;
-                          }
-                      }
-                    __syncthreads ();
-                    if (more)
-                      {
-                        stash ();
-                        __syncthreads ();
-                      }
-                  }
-                /* fold the diagonals into PS slot-aligned contributions per element */
-                const int s0 = (8 * G * (ga + gb)) >> 5;
-#pragma unroll
-                for (int v = 0; v < 4; v++)
-                  {
-                    i64 c[PS];
-#pragma unroll
-                    for (int q = 0; q < PS; q++) c[q] = 0;
-#pragma unroll
-                    for (int d = 0; d < ND; d++) c[d >> 2] += (i64) acc[d][v] << (8 * (d & 3));
-                    i64 *sl = my + v * slots + s0;
-                    i64 old[PS];
-#pragma unroll
-                    for (int q = 0; q < PS; q++) old[q] = sl[q];
-#pragma unroll
-                    for (int q = 0; q < PS; q++) sl[q] = old[q] + c[q];
-                  }
-              }
-          }
-      /* carry-normalise each element's slots into 32-bit words (all slot
-       * loads in flight at once), stage them in LDS and round from there */
-#pragma unroll 1
-      for (int v = 0; v < 4; v++)
-        {
-          const i64 i = ti * 32 + wr * 16 + 4 * g + v, j = tj * 32 + wc * 16 + r0;
-          const i64 *sl = my + v * slots;
-          i64 x[EXACT_SLMAX];
-#pragma unroll
-          for (int q = 0; q < EXACT_SLMAX; q++) x[q] = (q < slots) ? sl[q] : 0;
-          u32 *wb = words + tid * EXACT_SLMAX;
-          i64 carry = 0;
-#pragma unroll
-          for (int q = 0; q < EXACT_SLMAX; q++)
-            {
-              const i64 y = x[q] + carry;
-              if (q < slots) wb[q] = (u32) y;
-              carry = y >> 32;
-            }
-          if (i < m && j < n)
-            {
-              const i64 ea = emaxA[i], eb = emaxB[j];
-              const i64 scale = (ea == EXACT_EMPTY || eb == EXACT_EMPTY) ? 0 : ea + eb - sA - sB - 2 * (i64) PB;
-              round_store_int (wb, slots, scale, Cl, Cse, m * n, i * n + j);
-            }
-        }
+      const i64 y = x[q] + carry;
+      if (q < slots) wb[q] = (u32) y;
+      carry = y >> 32;
     }
+  const i64 ea = emaxA[i], eb = emaxB[j];
+  const i64 scale = (ea == EXACT_EMPTY || eb == EXACT_EMPTY) ? 0 : ea + eb - sA - sB - 2 * (i64) PB;
+  round_store_int (wb, slots, scale, Cl, Cse, m * n, e);
 }
 
 /* VALU fallback: one lane per output, the exact sum accumulated in `nw`
@@ -386,20 +240,13 @@ int exact_digits_launch (i64 lines, i64 len, i64 ls, i64 ps, const u64 *l, const
   return launch_status ("gemm_exact digits");
 }
 
-int exact_mfma_launch (int G, const exact_plan &p, const int8_t *DA, const int8_t *DB, const i64 *emaxA,
-                       const i64 *emaxB, i64 *slotbuf, int grid, u64 *Cl, i64 *Cse, hipStream_t s)
+int exact_round_launch (i64 m, i64 n, i64 np, i64 nE, int slots, const i64 *S64, const i64 *emaxA, const i64 *emaxB,
+                        i64 sA, i64 sB, u64 *Cl, i64 *Cse, hipStream_t s)
 {
-#define EXACT_G(GG)                                                                                        \
-  if (G == GG)                                                                                             \
-    {                                                                                                      \
-      hipLaunchKernelGGL ((k_exact_lds<GG>), dim3 (grid), dim3 (256), 0, s, p.m, p.n, p.mp, p.np, p.kp,    \
-                          p.nG, p.S, p.slots, DA, DB, emaxA, emaxB, p.sA, p.sB, slotbuf, Cl, Cse);         \
-      return launch_status ("gemm_exact mfma");                                                            \
-    }
-  EXACT_G (16)
-  EXACT_G (20)
-#undef EXACT_G
-  return MPGFX_EINVAL;
+  const i64 tot = m * n;
+  hipLaunchKernelGGL (k_exact_round, dim3 ((unsigned) ((tot + 255) / 256)), dim3 (256), 0, s, m, n, np, nE, slots, S64,
+                      emaxA, emaxB, sA, sB, Cl, Cse);
+  return launch_status ("gemm_exact round");
 }
 
 int exact_fallback_launch (i64 m, i64 n, i64 k, i64 r0, i64 rows, const u64 *Al, const i64 *Ase, const u64 *Bl,

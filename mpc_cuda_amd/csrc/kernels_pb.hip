@@ -333,7 +333,7 @@ int unpack (int cplx, i64 n, const u64 *l, const i64 *se, void *aos, hipStream_t
 #include "batch_pb.h"
 
 const pb_ops ops = { PB, ew_real, ew_cplx, axpy_real, axpy_cplx, gemm_perop, gemm_cplx, gemv, pack, unpack,
-                     exact_digits_launch, exact_mfma_launch, exact_fallback_launch, batch_launch };
+                     exact_digits_launch, exact_round_launch, exact_fallback_launch, batch_launch };
 
 } // namespace
 
