@@ -34,7 +34,8 @@
 
 namespace mpgfx {
 
-constexpr int EXACT_SMAX = 64;          /* digits per operand on the MFMA path */
+constexpr int EXACT_SMAX = 80;          /* digits per operand on the MFMA path */
+constexpr int EXACT_SPAD_MAX = 96;      /* digits incl. the zero planes of the last group */
 constexpr i64 EXACT_EMPTY = INT64_MIN;  /* row/column without nonzero entries */
 constexpr int EXACT_KCHUNK = 4096;      /* int32 flush period in k (G <= 20) */
 
@@ -42,9 +43,8 @@ inline int exact_digits (int pb, i64 spread) { return (int) ((pb + spread + 2 + 
 inline i64 pad_to (i64 x, i64 m) { return (x + m - 1) / m * m; }
 /* carry-save slots per output element for S digits (diagonals up to 2S-2) */
 inline int exact_slots (int S) { return (8 * (2 * S - 1) + 31) / 32 + 2; }
-/* slots for the largest padded digit count (pad_to(EXACT_SMAX, 16) digits) */
-constexpr int EXACT_SLMAX = (8 * (2 * 64 - 1) + 31) / 32 + 2;
-static_assert (EXACT_SMAX <= 64, "EXACT_SLMAX assumes at most 64 padded digits");
+/* slots for the largest padded digit count */
+constexpr int EXACT_SLMAX = (8 * (2 * EXACT_SPAD_MAX - 1) + 31) / 32 + 2;
 
 /* per-launch plan, filled on the host after the scan */
 struct exact_plan

@@ -227,9 +227,10 @@ int exact_mfma_run (const exact_plan &p, const int8_t *DA, const int8_t *DB, i64
                           DA, DB, S64);                                                                     \
       return launch_status ("gemm_exact mfma");                                                             \
     }
-  EXACT_G (8)
-  EXACT_G (12)
-  EXACT_G (16)
+  EXACT_G (9)
+  EXACT_G (11)
+  EXACT_G (13)
+  EXACT_G (15)
   EXACT_G (17)
 #undef EXACT_G
   return MPGFX_EINVAL;
@@ -249,21 +250,20 @@ static i64 exact_grid (i64 tiles)
   return (g + 7) / 8 * 8;
 }
 
-/* digit groups: G in {8, 12, 16, 17} and nG = ceil(S/G) with the padded digit
- * count G*nG <= 64 (zero planes past S); least padded work (G*nG)^2 first,
- * then fewest passes */
+/* digit groups: G in {9, 11, 13, 15, 17}, nG = ceil(S/G), zero planes past S
+ * up to G*nG <= EXACT_SPAD_MAX.  Cost model: padded MFMA work (G*nG)^2, times
+ * (1 + 4/G) for the staging and flush overhead of a pass (a pass stages 2G
+ * planes per k step for G^2 MFMAs). */
 static void choose_groups (int S, int *G, int *nG)
 {
-  const int cand[4] = {8, 12, 16, 17};
-  long best = -1;
-  int best_passes = 0;
+  const int cand[5] = {9, 11, 13, 15, 17};
+  double best = -1;
   for (int c : cand)
     {
       const int ng = (S + c - 1) / c;
-      if (ng * c > 64) continue;
-      const long work = (long) (ng * c) * (ng * c);
-      if (best < 0 || work < best || (work == best && ng * ng < best_passes))
-        { best = work; best_passes = ng * ng; *G = c; *nG = ng; }
+      if (ng * c > EXACT_SPAD_MAX) continue;
+      const double cost = (double) (ng * c) * (ng * c) * (1.0 + 4.0 / c);
+      if (best < 0 || cost < best) { best = cost; *G = c; *nG = ng; }
     }
 }
 
@@ -272,7 +272,7 @@ size_t exact_workspace_bound (int pb, i64 m, i64 n, i64 k)
 {
   (void) pb;
   const i64 mp = pad_to (m, 32), np = pad_to (n, 32), kp = pad_to (k > 0 ? k : 1, 64);
-  const i64 spad = 64;                                   /* G*nG <= 64 */
+  const i64 spad = EXACT_SPAD_MAX;                       /* G*nG */
   size_t b = 0;
   b += (size_t) (2 * m + 2 * n + 8) * 8;                 /* scan outputs + spread words */
   b = (size_t) pad_to ((i64) b, 256);
