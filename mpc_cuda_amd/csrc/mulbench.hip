@@ -87,7 +87,7 @@ __global__ __launch_bounds__ (256) void k_generic (int pairs, int reps, int w, c
 }
 
 /* INT8 MFMA peak: 8 independent v_mfma_i32_16x16x64_i8 accumulator chains
- * per wavefront (16*16*64*2 = 32768 ops each) */
+ * per wavefront (16*16*64*2 = 32768 ops each), operands fixed in registers */
 typedef int v4i __attribute__ ((ext_vector_type (4)));
 __global__ __launch_bounds__ (256) void k_mfma_peak (int iters, int seed, int *out)
 {
@@ -100,7 +100,6 @@ __global__ __launch_bounds__ (256) void k_mfma_peak (int iters, int seed, int *o
     {
 #pragma unroll
       for (int j = 0; j < 8; j++) acc[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8 (a, b, acc[j], 0, 0, 0);
-      a.x ^= it;
     }
   int x = 0;
 #pragma unroll
