@@ -39,8 +39,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--extra", action="store_true", help="also time the other configurations")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--traffic-driver", action="store_true",
-                    help="only run 3 headline launches and 3 calibration copies (for rocprofv3 --pmc)")
+    ap.add_argument("--driver", choices=["axpy", "exact", "gemv"], default=None,
+                    help="only run a few launches of one kernel (for rocprofv3 --pmc): the headline AXPY "
+                         "plus calibration copies, the exact GEMM at 1024^3, or the GEMV at 4096^2")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -51,8 +52,8 @@ def main():
     import mpc_cuda_amd as M
 
     torch.cuda.set_device(0)
-    if args.traffic_driver:
-        traffic_driver()
+    if args.driver:
+        profile_driver(args.driver)
         return
     a = M.random_complex(PB, 1, 201, -16, 16).cuda()
     x = M.random_complex(PB, N_ELEM, 202, -16, 16).cuda()
@@ -111,23 +112,39 @@ def main():
     print(json.dumps(rec))
 
 
-def traffic_driver():
-    """The launches profiled for profiles/axpy_cplx_pb1024_traffic.md: three
-    headline AXPYs on device-generated inputs, then three 512 MiB device
-    copies whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
+def profile_driver(which):
+    """Launches for the rocprofv3 --pmc passes summarised under profiles/:
+    "axpy": three headline AXPYs on device-generated inputs, then three 512 MiB
+    device copies whose FETCH_SIZE / WRITE_SIZE calibrate the counters;
+    "exact": three exact GEMMs pb 256 1024^3 (exponents in [-4, 4]);
+    "gemv": three per-op GEMVs pb 512 4096^2."""
     import torch
     import mpc_cuda_amd as M
-    a = M.device_random(PB, 1, 1, -16, 16, cplx=True)
-    x = M.device_random(PB, N_ELEM, 2, -16, 16, cplx=True)
-    y = M.device_random(PB, N_ELEM, 3, -16, 16, cplx=True)
-    for _ in range(3):
-        M.axpy(a, x, y)
-    src = torch.empty(1 << 29, dtype=torch.uint8, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
+    if which == "axpy":
+        a = M.device_random(PB, 1, 1, -16, 16, cplx=True)
+        x = M.device_random(PB, N_ELEM, 2, -16, 16, cplx=True)
+        y = M.device_random(PB, N_ELEM, 3, -16, 16, cplx=True)
+        for _ in range(3):
+            M.axpy(a, x, y)
+        src = torch.empty(1 << 29, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+    elif which == "exact":
+        d = 1024
+        A = M.device_random(256, d * d, 401, -4, 4)
+        B = M.device_random(256, d * d, 402, -4, 4)
+        C = M.empty(256, d * d)
+        for _ in range(3):
+            M.gemm(A, B, d, d, d, "exact", out=C)
+    else:
+        m = k = 4096
+        A = M.device_random(512, m * k, 301, -4, 4)
+        x = M.device_random(512, k, 302, -4, 4)
+        for _ in range(3):
+            M.gemv(A, x, m, k)
     torch.cuda.synchronize()
-    print("traffic driver done", flush=True)
+    print(f"profile driver {which} done", flush=True)
 
 
 def measured_traffic():
