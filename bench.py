@@ -21,6 +21,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -45,7 +46,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
-              "the path's multi-GPU sharding is described in DESIGN.md but not run", file=sys.stderr)
+              "the path's sharding over several GPUs is designed but not run", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -303,7 +304,8 @@ def batch_config(args, n_tasks=10000, seed=5, check=50):
     torch.cuda.synchronize()
     res = {"instances": n_tasks, "mp_ops": mp_ops, "word_products": wp}
     for label, tuning in (("fixed", False), ("tuned", True)):
-        bt = B.Batch(tasks, tuning=tuning, cache=os.path.join(ROOT, "gpurun_out", "batch_tune.json") if tuning else None)
+        bt = B.Batch(tasks, tuning=tuning,
+                     cache=os.path.join(tempfile.gettempdir(), "mpgfx_batch_tune.json") if tuning else None)
         ms = _timed(bt.run, 3)
         res[label] = {"ms": ms, "instances_per_s": n_tasks / (ms * 1e-3), "MP_ops_per_s": mp_ops / (ms * 1e-3),
                       "word_products_per_s": wp / (ms * 1e-3), "scratch_GB": bt.info()["scratch_bytes"] / 1e9}
