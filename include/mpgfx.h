@@ -137,6 +137,18 @@ int mpgfx_gemm_complex (int pb, int64_t m, int64_t n, int64_t k,
                         const uint64_t *B_limbs, const int64_t *B_se,
                         uint64_t *C_limbs, int64_t *C_se, void *stream);
 
+/* Complex exact GEMM: each component of C = A*B is the exact sum of the exact
+ * products, rounded once (Re = RNDN(sum ar*br - ai*bi), Im = RNDN(sum ar*bi +
+ * ai*br)).  No counterpart in the reference: the exact mode of
+ * mpgfx_gemm_real (mpfr_dot semantics, src/mpfr_cuda/dot.c:47-54) carried to
+ * the complex matmul of demos/matmul_mpc.cu:33-52.  Runs as two real exact
+ * GEMMs over 2k terms on INT8 MFMA; allocates its operand copies and
+ * workspace stream-ordered and synchronises the stream (the exponent scans). */
+int mpgfx_gemm_complex_exact (int pb, int64_t m, int64_t n, int64_t k,
+                              const uint64_t *A_limbs, const int64_t *A_se,
+                              const uint64_t *B_limbs, const int64_t *B_se,
+                              uint64_t *C_limbs, int64_t *C_se, void *stream);
+
 /* Convert between arrays of the reference's AoS structs (cu_freal<PB>:
  * {int sign; long exp; cu_limb m[N];}, sizeof 16+8N, cu_freal.cuh:75-77;
  * cu_fcomplex<PB>: {re, im}, cu_fcomplex.cuh:102) and MP arrays. `aos` is a

@@ -132,6 +132,18 @@ extern "C" int mpgfx_gemm_complex (int pb, int64_t m, int64_t n, int64_t k, cons
   return o->gemm_cplx (m, n, k, Al, Ase, Bl, Bse, Cl, Cse, (hipStream_t) stream);
 }
 
+extern "C" int mpgfx_gemm_complex_exact (int pb, int64_t m, int64_t n, int64_t k, const uint64_t *Al,
+                                         const int64_t *Ase, const uint64_t *Bl, const int64_t *Bse, uint64_t *Cl,
+                                         int64_t *Cse, void *stream)
+{
+  if (m < 0 || n < 0 || k < 0) return MPGFX_EINVAL;
+  GET_OPS (o);
+  if (m == 0 || n == 0) return MPGFX_OK;
+  NEED (Cl && Cse);
+  if (k > 0) NEED (Al && Ase && Bl && Bse);
+  return gemm_exact_complex (o, pb, m, n, k, Al, Ase, Bl, Bse, Cl, Cse, (hipStream_t) stream);
+}
+
 extern "C" int mpgfx_pack (int pb, int cplx, int64_t n, const void *aos, uint64_t *l, int64_t *se, void *stream)
 {
   if (n < 0) return MPGFX_EINVAL;

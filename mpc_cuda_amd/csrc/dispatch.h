@@ -47,6 +47,8 @@ const pb_ops *find_ops (int pb);
 size_t exact_workspace_bound (int pb, i64 m, i64 n, i64 k);
 int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl,
                 const i64 *Bse, u64 *Cl, i64 *Cse, void *ws, size_t ws_bytes, hipStream_t s);
+int gemm_exact_complex (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl,
+                        const i64 *Bse, u64 *Cl, i64 *Cse, hipStream_t s);
 
 } // namespace mpgfx
 #endif

@@ -368,4 +368,87 @@ int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, con
   return done (rc);
 }
 
+/* sign flip of every value of a real MP array (se ^= 1; zeros are skipped by
+ * the exact path whatever their sign field) */
+__global__ __launch_bounds__ (256) void k_negate (i64 n, const i64 *__restrict__ src, i64 *__restrict__ dst)
+{
+  for (i64 i = (i64) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64) gridDim.x * blockDim.x)
+    dst[i] = src[i] ^ 1;
+}
+
+/* Complex exact GEMM: Re C = RNDN(sum_k ar br - ai bi), Im C = RNDN(sum_k
+ * ar bi + ai br), each component one exact sum rounded once.  Both are real
+ * exact GEMMs over 2k terms, Re = [Ar | Ai] [Br ; -Bi] and Im = [Ar | Ai]
+ * [Bi ; Br], so they run on the MFMA path above; the concatenated operands
+ * are assembled with stream-ordered device copies. */
+int gemm_exact_complex (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl,
+                        const i64 *Bse, u64 *Cl, i64 *Cse, hipStream_t s)
+{
+  const int NL = nlimbs (pb);
+  u64 *Cil = Cl + (size_t) NL * m * n;
+  i64 *Cis = Cse + m * n;
+  if (k == 0)
+    {
+      int rc = o->gemm_perop (m, n, 0, Al, Ase, Bl, Bse, Cl, Cse, s);
+      return rc ? rc : o->gemm_perop (m, n, 0, Al, Ase, Bl, Bse, Cil, Cis, s);
+    }
+  const i64 nA = m * k, nB = k * n, nA2 = 2 * nA, nB2 = 2 * nB;
+  const size_t bytes = (size_t) (nA2 + 2 * nB2) * (NL + 1) * 8;
+  char *buf = nullptr;
+  if (hip_ok ("gemm_exact complex operands", hipMallocAsync ((void **) &buf, bytes, s))) return MPGFX_ENOMEM;
+  u64 *A2l = (u64 *) buf;
+  i64 *A2s = (i64 *) (A2l + (size_t) NL * nA2);
+  u64 *Brl = (u64 *) (A2s + nA2);
+  i64 *Brs = (i64 *) (Brl + (size_t) NL * nB2);
+  u64 *Bil = (u64 *) (Brs + nB2);
+  i64 *Bis = (i64 *) (Bil + (size_t) NL * nB2);
+  const u64 *Arl = Al, *Ail = Al + (size_t) NL * nA;
+  const i64 *Ars = Ase, *Ais = Ase + nA;
+  const u64 *Brl0 = Bl, *Bil0 = Bl + (size_t) NL * nB;
+  const i64 *Brs0 = Bse, *Bis0 = Bse + nB;
+  int rc = MPGFX_OK;
+  /* A2 = [Ar | Ai]: each plane is m rows of 2k values */
+  for (int l = 0; l <= NL && !rc; l++)
+    {
+      const bool se = (l == NL);
+      void *dst = se ? (void *) A2s : (void *) (A2l + (size_t) l * nA2);
+      const void *sr = se ? (const void *) Ars : (const void *) (Arl + (size_t) l * nA);
+      const void *si = se ? (const void *) Ais : (const void *) (Ail + (size_t) l * nA);
+      rc = hip_ok ("gemm_exact complex copy", hipMemcpy2DAsync (dst, 2 * k * 8, sr, k * 8, k * 8, m,
+                                                                hipMemcpyDeviceToDevice, s));
+      if (!rc)
+        rc = hip_ok ("gemm_exact complex copy", hipMemcpy2DAsync ((char *) dst + k * 8, 2 * k * 8, si, k * 8, k * 8, m,
+                                                                  hipMemcpyDeviceToDevice, s));
+    }
+  /* Bre = [Br ; -Bi], Bim = [Bi ; Br]: row blocks, contiguous per plane */
+  for (int l = 0; l < NL && !rc; l++)
+    {
+      const size_t pl = (size_t) nB * 8;
+      rc = hip_ok ("gemm_exact complex copy", hipMemcpyAsync (Brl + (size_t) l * nB2, Brl0 + (size_t) l * nB, pl,
+                                                              hipMemcpyDeviceToDevice, s));
+      if (!rc) rc = hip_ok ("gemm_exact complex copy", hipMemcpyAsync (Brl + (size_t) l * nB2 + nB, Bil0 + (size_t) l * nB,
+                                                                       pl, hipMemcpyDeviceToDevice, s));
+      if (!rc) rc = hip_ok ("gemm_exact complex copy", hipMemcpyAsync (Bil + (size_t) l * nB2, Bil0 + (size_t) l * nB, pl,
+                                                                       hipMemcpyDeviceToDevice, s));
+      if (!rc) rc = hip_ok ("gemm_exact complex copy", hipMemcpyAsync (Bil + (size_t) l * nB2 + nB, Brl0 + (size_t) l * nB,
+                                                                       pl, hipMemcpyDeviceToDevice, s));
+    }
+  if (!rc)
+    {
+      rc = hip_ok ("gemm_exact complex copy", hipMemcpyAsync (Brs, Brs0, (size_t) nB * 8, hipMemcpyDeviceToDevice, s));
+      if (!rc) rc = hip_ok ("gemm_exact complex copy", hipMemcpyAsync (Bis, Bis0, (size_t) nB * 8, hipMemcpyDeviceToDevice, s));
+      if (!rc) rc = hip_ok ("gemm_exact complex copy", hipMemcpyAsync (Bis + nB, Brs0, (size_t) nB * 8, hipMemcpyDeviceToDevice, s));
+    }
+  if (!rc)
+    {
+      const i64 g = (nB + 255) / 256;
+      hipLaunchKernelGGL (k_negate, dim3 ((unsigned) (g < 65536 ? g : 65536)), dim3 (256), 0, s, nB, Bis0, Brs + nB);
+      rc = launch_status ("gemm_exact complex negate");
+    }
+  if (!rc) rc = gemm_exact (o, pb, m, n, 2 * k, A2l, A2s, Brl, Brs, Cl, Cse, nullptr, 0, s);
+  if (!rc) rc = gemm_exact (o, pb, m, n, 2 * k, A2l, A2s, Bil, Bis, Cil, Cis, nullptr, 0, s);
+  (void) hipFreeAsync (buf, s);
+  return rc;
+}
+
 } // namespace mpgfx

@@ -73,8 +73,8 @@ def gemv(A, x, m, k, out=None):
 
 def gemm(A, B, m, n, k, mode="perop", out=None, workspace=None):
     """C = A B (A: m*k, B: k*n values, row-major).  mode 'perop' (the
-    reference's matmul order) or 'exact' (exact sum, one rounding); complex
-    arrays use the per-op complex GEMM."""
+    reference's matmul order) or 'exact' (exact sum, one rounding per real
+    value or per complex component)."""
     _dev(A), _dev(B)
     _same(A.pb, B)
     if A.n != m * k or B.n != k * n or A.cplx != B.cplx:
@@ -82,10 +82,14 @@ def gemm(A, B, m, n, k, mode="perop", out=None, workspace=None):
     out = out if out is not None else empty(A.pb, m * n, A.cplx, device=A.se.device)
     L = _lib.lib()
     if A.cplx:
-        if mode != "perop":
-            raise ValueError("complex GEMM supports mode='perop'")
-        _lib.check("mpgfx_gemm_complex",
-                   L.mpgfx_gemm_complex(A.pb, m, n, k, *A.ptrs(), *B.ptrs(), *out.ptrs(), _stream()))
+        if mode == "perop":
+            _lib.check("mpgfx_gemm_complex",
+                       L.mpgfx_gemm_complex(A.pb, m, n, k, *A.ptrs(), *B.ptrs(), *out.ptrs(), _stream()))
+        elif mode == "exact":
+            _lib.check("mpgfx_gemm_complex_exact",
+                       L.mpgfx_gemm_complex_exact(A.pb, m, n, k, *A.ptrs(), *B.ptrs(), *out.ptrs(), _stream()))
+        else:
+            raise ValueError(mode)
         return out
     md = {"perop": _lib.GEMM_PEROP, "exact": _lib.GEMM_EXACT}[mode]
     ws_ptr, ws_bytes = None, 0

@@ -161,9 +161,11 @@ def axpy_complex(pb, a, x, y, impl="oracle"):
 
 def gemm(pb, m, n, k, A, B, mode="perop", rows=None, impl="oracle"):
     """C = A*B with A m x k, B k x n (row-major element order).  mode: perop |
-    exact | cperop (complex per-op).  rows=(r0, r1) limits the computed rows;
+    exact | cperop (complex per-op) | cexact (complex, exact per component).  rows=(r0, r1) limits the computed rows;
     the other rows of the result stay zero."""
     r0, r1 = rows if rows is not None else (0, m)
+    if mode == "cexact":
+        return _cgemm_exact(pb, m, n, k, A, B, rows)
     if mode == "cperop":
         Cl, Cse = empty_complex(pb, m * n)
     else:
@@ -182,6 +184,23 @@ def gemm(pb, m, n, k, A, B, mode="perop", rows=None, impl="oracle"):
     else:
         raise ValueError(mode)
     return Cl, Cse
+
+
+def _cgemm_exact(pb, m, n, k, A, B, rows):
+    """Complex exact GEMM: Re C = RNDN(sum_k ar*br - ai*bi), Im C = RNDN(sum_k
+    ar*bi + ai*br), each component one exact sum rounded once.  Computed as two
+    real exact GEMMs over 2k terms: Re = [Ar | Ai] [Br ; -Bi], Im = [Ar | Ai]
+    [Bi ; Br] (negation flips the sign bit)."""
+    (Al, As), (Bl, Bs) = A, B
+    N = nlimbs(pb)
+    A2l = np.concatenate([Al[0].reshape(N, m, k), Al[1].reshape(N, m, k)], axis=2).reshape(N, 2 * m * k)
+    A2s = np.concatenate([As[0].reshape(m, k), As[1].reshape(m, k)], axis=1).reshape(2 * m * k)
+    Bre = (np.concatenate([Bl[0], Bl[1]], axis=1), np.concatenate([Bs[0], Bs[1] ^ 1]))
+    Bim = (np.concatenate([Bl[1], Bl[0]], axis=1), np.concatenate([Bs[1], Bs[0]]))
+    A2 = (np.ascontiguousarray(A2l), np.ascontiguousarray(A2s))
+    re = gemm(pb, m, n, 2 * k, A2, tuple(np.ascontiguousarray(v) for v in Bre), "exact", rows)
+    im = gemm(pb, m, n, 2 * k, A2, tuple(np.ascontiguousarray(v) for v in Bim), "exact", rows)
+    return np.stack([re[0], im[0]]), np.stack([re[1], im[1]])
 
 
 # ---------------------------------------------------------------- MPFR driver

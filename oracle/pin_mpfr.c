@@ -16,6 +16,8 @@
  *       caxpy                  (n = M)        complex a[1], x, y -> o
  *       gemm_perop gemm_exact  (M N K)        A[M*K], B[K*N]    -> C[M*N]
  *       cgemm_perop            (M N K)        complex A, B      -> C
+ *       cgemm_exact            (M N K)        complex A, B      -> C  (each component:
+ *                                               exact products, one mpfr_sum) 
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -263,6 +265,53 @@ int main (int argc, char **argv)
               cput (acc, C, M * Nn, i * Nn + j);
             }
         mpc_clear (acc); mpc_clear (x); mpc_clear (y); mpc_clear (t);
+      }
+      write_arr (fo, C);
+    }
+  else if (!strcmp (op, "cgemm_exact"))
+    {
+      /* Re C = RNDN(sum_k ar*br - ai*bi), Im C = RNDN(sum_k ar*bi + ai*br): every
+       * product exact at 2*PB bits, one correctly rounded sum per component */
+      arr A = read_arr (fi, 2 * M * K), B = read_arr (fi, 2 * K * Nn), C = new_arr (2 * M * Nn);
+#pragma omp parallel
+      {
+        mpfr_t acc, ar, ai, br, bi;
+        mpfr_inits2 (pb, acc, ar, ai, br, bi, (mpfr_ptr) 0);
+        const int64_t P = 2 * (K ? K : 1);
+        mpfr_t *prod = malloc (sizeof (mpfr_t) * (size_t) P);
+        mpfr_ptr *tab = malloc (sizeof (mpfr_ptr) * (size_t) P);
+        for (int64_t q = 0; q < P; q++) { mpfr_init2 (prod[q], 2 * pb); tab[q] = prod[q]; }
+#pragma omp for schedule(static) collapse(2)
+        for (int64_t i = 0; i < M; i++)
+          for (int64_t j = 0; j < Nn; j++)
+            for (int part = 0; part < 2; part++)
+              {
+                for (int64_t k = 0; k < K; k++)
+                  {
+                    const int64_t ia = i * K + k, ib = k * Nn + j;
+                    to_mpfr (ar, A.l, A.se, M * K, ia);
+                    to_mpfr (ai, A.l + (int64_t) N * M * K, A.se + M * K, M * K, ia);
+                    to_mpfr (br, B.l, B.se, K * Nn, ib);
+                    to_mpfr (bi, B.l + (int64_t) N * K * Nn, B.se + K * Nn, K * Nn, ib);
+                    if (part == 0)
+                      {
+                        if (mpfr_mul (prod[2 * k], ar, br, MPFR_RNDN) != 0) die ("inexact product");
+                        if (mpfr_mul (prod[2 * k + 1], ai, bi, MPFR_RNDN) != 0) die ("inexact product");
+                        mpfr_neg (prod[2 * k + 1], prod[2 * k + 1], MPFR_RNDN);
+                      }
+                    else
+                      {
+                        if (mpfr_mul (prod[2 * k], ar, bi, MPFR_RNDN) != 0) die ("inexact product");
+                        if (mpfr_mul (prod[2 * k + 1], ai, br, MPFR_RNDN) != 0) die ("inexact product");
+                      }
+                  }
+                if (K == 0) mpfr_set_zero (acc, 1);
+                else mpfr_sum (acc, tab, (unsigned long) (2 * K), MPFR_RNDN);
+                from_mpfr (acc, C.l + (int64_t) part * N * M * Nn, C.se + (int64_t) part * M * Nn, M * Nn, i * Nn + j);
+              }
+        for (int64_t q = 0; q < P; q++) mpfr_clear (prod[q]);
+        free (prod); free (tab);
+        mpfr_clears (acc, ar, ai, br, bi, (mpfr_ptr) 0);
       }
       write_arr (fo, C);
     }

@@ -96,6 +96,9 @@ def main():
                   ((2, N(pb), m * nn), (2, m * nn)), [m, nn, k])
         out.update({"cgA_limbs": CA.limbs, "cgA_se": CA.se, "cgB_limbs": CB.limbs, "cgB_se": CB.se,
                     "cgperop_limbs": r[0], "cgperop_se": r[1]})
+        r = O.pin("cgemm_exact", pb, [(CA.limbs, CA.se), (CB.limbs, CB.se)],
+                  ((2, N(pb), m * nn), (2, m * nn)), [m, nn, k])
+        out.update({"cgexact_limbs": r[0], "cgexact_se": r[1]})
         out["dims"] = np.array([m, nn, k], np.int64)
         path = os.path.join(HERE, f"pb{pb}.npz")
         np.savez_compressed(path, **out)

@@ -13,8 +13,8 @@ import goldens
 pytestmark = pytest.mark.gpu
 
 
-def dev(pair, pb):
-    return MPArray(pb, np.ascontiguousarray(pair[0]), np.ascontiguousarray(pair[1])).cuda()
+def dev(pair, pb, cplx=False):
+    return MPArray(pb, np.ascontiguousarray(pair[0]), np.ascontiguousarray(pair[1]), cplx).cuda()
 
 
 @pytest.mark.parametrize("pb", goldens.PBS)
@@ -93,3 +93,42 @@ def test_exact_gemm_256_full_size_sampled(gpu, lo, hi):
         sl = slice(r0 * n, (r0 + 1) * n)
         assert np.array_equal(C.limbs[:, sl], ref[0][:, sl]), r0
         assert np.array_equal(C.se[sl], ref[1][sl]), r0
+
+
+@pytest.mark.parametrize("pb", goldens.PBS)
+def test_complex_exact_gemm_vs_golden(gpu, pb):
+    """Complex exact GEMM (each component one exact sum, one rounding): by value
+    against MPFR's mpfr_sum fixture, bit for bit against the oracle."""
+    g = goldens.load(pb)
+    m, n, k = (int(v) for v in g["dims"])
+    A, B = goldens.pair(g, "cgA"), goldens.pair(g, "cgB")
+    r = M.gemm(dev(A, pb, True), dev(B, pb, True), m, n, k, "exact")
+    assert O.mismatches(r.pair(), goldens.pair(g, "cgexact"), ignore_zero_sign=True) == 0
+    assert O.mismatches(r.pair(), O.gemm(pb, m, n, k, A, B, "cexact"), ignore_zero_sign=False) == 0
+
+
+@pytest.mark.parametrize("pb,m,n,k,lo,hi", [(256, 40, 33, 300, -4, 4), (96, 17, 20, 64, -60, 60), (512, 8, 9, 70, -8, 8)])
+def test_complex_exact_gemm_vs_oracle(gpu, pb, m, n, k, lo, hi):
+    A = M.random_complex(pb, m * k, 91 + pb, lo, hi)
+    B = M.random_complex(pb, k * n, 92 + pb, lo, hi)
+    r = M.gemm(A.cuda(), B.cuda(), m, n, k, "exact")
+    assert O.mismatches(r.pair(), O.gemm(pb, m, n, k, A.pair(), B.pair(), "cexact"), ignore_zero_sign=False) == 0
+
+
+def test_complex_exact_gemm_cancellation(gpu):
+    """ai = ar and bi = br: the real parts cancel exactly (+0); then one bit of
+    one imaginary entry changed: near-total cancellation."""
+    pb, m, n, k = 256, 7, 6, 33
+    A = M.random_complex(pb, m * k, 95, -20, 20)
+    B = M.random_complex(pb, k * n, 96, -20, 20)
+    Al, As = (np.array(v) for v in A.pair())
+    Bl, Bs = (np.array(v) for v in B.pair())
+    Al[1], As[1] = Al[0], As[0]
+    Bl[1], Bs[1] = Bl[0], Bs[0]
+    for flip in (False, True):
+        if flip:
+            Bl[1][0] ^= np.uint64(1 << 40)
+        a = (np.ascontiguousarray(Al), np.ascontiguousarray(As))
+        b = (np.ascontiguousarray(Bl), np.ascontiguousarray(Bs))
+        r = M.gemm(dev(a, pb, True), dev(b, pb, True), m, n, k, "exact")
+        assert O.mismatches(r.pair(), O.gemm(pb, m, n, k, a, b, "cexact"), ignore_zero_sign=False) == 0, flip

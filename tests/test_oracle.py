@@ -38,6 +38,34 @@ def test_oracle_vs_golden(pb):
             assert O.mismatches(O.gemm(pb, m, n, k, A, B, mode), goldens.pair(g, tag + mode)) == 0, (tag, mode)
     r = O.gemm(pb, m, n, k, goldens.pair(g, "cgA"), goldens.pair(g, "cgB"), "cperop")
     assert O.mismatches(r, goldens.pair(g, "cgperop")) == 0
+    r = O.gemm(pb, m, n, k, goldens.pair(g, "cgA"), goldens.pair(g, "cgB"), "cexact")
+    assert O.mismatches(r, goldens.pair(g, "cgexact")) == 0
+
+
+@pytest.mark.skipif(not O.pin_available(), reason="GMP/MPFR/MPC not available (oracle/build/pin_mpfr)")
+@pytest.mark.parametrize("pb", [64, 96, 256, 512])
+def test_oracle_complex_exact_gemm_vs_mpfr(pb):
+    """Complex exact GEMM (each component one exact sum, one rounding) against
+    MPFR's mpfr_sum over the exact products, on wide exponent spreads and on
+    inputs whose real parts cancel exactly (ai = ar, bi = br: Re = 0) or almost
+    (bi = br with the last limb changed)."""
+    from mpc_cuda_amd import gen as G
+    m, n, k = 6, 5, 19
+    A = G.random_complex(pb, m * k, 61 + pb, -80, 80)
+    B = G.random_complex(pb, k * n, 62 + pb, -80, 80)
+    cases = [(A.pair(), B.pair())]
+    Al, As = (np.array(v) for v in A.pair())
+    Bl, Bs = (np.array(v) for v in B.pair())
+    Al[1], As[1] = Al[0], As[0]
+    Bl[1], Bs[1] = Bl[0], Bs[0]
+    cases.append(((Al.copy(), As.copy()), (Bl.copy(), Bs.copy())))
+    Bl[1][0] ^= np.uint64(1 << 40)                  # inside the kept bits at every pb here
+    cases.append(((Al, As), (Bl, Bs)))
+    for ca, cb in cases:
+        ca = tuple(np.ascontiguousarray(v) for v in ca)
+        cb = tuple(np.ascontiguousarray(v) for v in cb)
+        ref = O.pin("cgemm_exact", pb, [ca, cb], ((2, O.nlimbs(pb), m * n), (2, m * n)), [m, n, k])
+        assert O.mismatches(O.gemm(pb, m, n, k, ca, cb, "cexact"), ref) == 0
 
 
 REF_SEEDS = {32: 0x1111, 64: 0x2222, 96: 0x3333, 128: 0x4444, 160: 0x5555, 256: 0x6666, 288: 0x7777,
