@@ -26,6 +26,11 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The batched execution's hipGraph runs one branch per (precision, real/complex)
+# class; the HIP runtime maps concurrent branches onto its hardware queues (4
+# by default), so allow enough queues for the classes to run side by side.
+# Read by the runtime at initialisation, hence before anything imports torch.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "12")
 
 HBM_PEAK_GBS = 8000.0            # MI355X HBM3E, published
 PB, N_ELEM = 1024, 1 << 18
@@ -228,7 +233,8 @@ def side_metrics():
 def extra_configs(args):
     """Every other configuration, on GPU-generated inputs: real AXPY over the
     precision sweep (n = 2^20), GEMV 4096^2 at 512 bits, the exact GEMM with
-    exponent spread [-80, 80], and the 10,000-instance batched workload."""
+    exponent spread [-80, 80], the complex GEMM 1024^3 at 256 bits (per-op and
+    exact), and the 10,000-instance batched workload."""
     import mpc_cuda_amd as M
     out = {}
     for pb in (64, 128, 512, 1024, 2048):
@@ -254,6 +260,15 @@ def extra_configs(args):
     B = M.device_random(pb, d * d, 502, -80, 80)
     ms = _timed(lambda: M.gemm(A, B, d, d, d, "exact"), 2)
     out["gemm_exact_pb256_1024^3_spread80"] = {"ms": ms, "MP_ops_per_s": 2 * d ** 3 / (ms * 1e-3)}
+    del A, B
+    # complex GEMM pb 256, 1024^3: per-op (MPC_RNDNN per operation) vs exact
+    # per component (one rounding); one complex multiply-add = 2 MP ops
+    A = M.device_random(pb, d * d, 511, -4, 4, cplx=True)
+    B = M.device_random(pb, d * d, 512, -4, 4, cplx=True)
+    ms_p = _timed(lambda: M.gemm(A, B, d, d, d, "perop"), 1)
+    ms_e = _timed(lambda: M.gemm(A, B, d, d, d, "exact"), 2)
+    out["cgemm_pb256_1024^3"] = {"perop_ms": ms_p, "exact_ms": ms_e, "exact_speedup": ms_p / ms_e,
+                                 "exact_MP_ops_per_s": 2 * d ** 3 / (ms_e * 1e-3)}
     del A, B
     out["batch_10000"] = batch_config(args)
     return out

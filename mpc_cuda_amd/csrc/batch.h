@@ -10,7 +10,10 @@
  *           wavefront folds one group, coop_lanes(pb) lanes per chain.  The
  *           product scratch holds records of N limbs + the sign/exponent word;
  *           step s of the chain at position p is record gbase[p/R] + R*s + p%R,
- *           so a wavefront's loads at step s are contiguous.
+ *           so a wavefront's loads at step s are contiguous.  The fold units of
+ *           all classes (one per group, per part for complex classes) form one
+ *           list sorted by length, folded by a single kernel so the longest
+ *           chains of every class start first and run side by side.
  */
 #ifndef MPGFX_BATCH_H
 #define MPGFX_BATCH_H
@@ -44,6 +47,7 @@ struct bt_chain
 
 struct bt_class
 {
+  int32_t pb, cplx;
   const bt_task *tasks;
   const bt_item *items;
   i64 nitems;
@@ -53,6 +57,14 @@ struct bt_class
   u64 *scr;                                  /* product records (re part, then im part) */
   i64 scr_n;                                /* records per part */
   unsigned *counters;                       /* [0] phase-1 queue, [1] phase-2 queue */
+};
+
+/* one wavefront's work in phase 2: fold group fg (part 0 real / 1 imaginary)
+ * of class cls */
+struct bt_unit
+{
+  int32_t cls, part;
+  i64 fg;
 };
 
 } // namespace mpgfx

@@ -3,17 +3,19 @@
  *
  * Planning groups the tasks into classes by (precision, real/complex), builds
  * each class's phase-1 item list and its phase-2 chain groups (sorted longest
- * first, 64 per wavefront), sizes the product scratch for the batch, and
- * captures, per class, [reset counters -> product kernel -> fold kernel] on a
- * forked stream of a single hipGraph, so classes run concurrently and one
- * graph launch runs the whole batch.  The counters are reset by a small kernel
- * inside the graph (no memset nodes).
+ * first, R chains per wavefront), sizes the product scratch for the batch, and
+ * captures a single hipGraph: per class [reset counters -> product kernel] on
+ * a forked stream, then, after the join, one fold kernel for the fold units of
+ * every class (sorted longest first across classes).  One graph launch runs
+ * the whole batch.  Counters are reset by small kernels inside the graph (no
+ * memset nodes).
  *
- * Tuning: per (phase, precision, real/complex, length bucket) the mapping
- * (values per product item, lanes per workgroup, workgroups per CU) is either
- * the fixed default or the tuner's pick: each candidate is timed on the
- * batch's own data, and the winners are cached in a small JSON file keyed by
- * that tuple.
+ * Tuning: the mapping (values per product item, lanes per workgroup,
+ * workgroups per CU) of each class's products (keyed by precision,
+ * real/complex and length bucket) and of the fold kernel (keyed by the number
+ * of units and the longest chain, as log2 buckets) is either the fixed default
+ * or the tuner's pick: each candidate is timed on the batch's own data, and
+ * the winners are cached in a small JSON file.
  */
 #include "common.h"
 #include "dispatch.h"
@@ -38,10 +40,16 @@ struct mpgfx_batch
     void *d_tasks = nullptr, *d_items = nullptr, *d_chains = nullptr, *d_gbase = nullptr;
     void *d_scr_l = nullptr, *d_scr_s = nullptr, *d_counters = nullptr;
     int64_t total_values = 0, total_chain_steps = 0, nchains = 0;
-    mpgfx_mapping map1, map2;
-    int bucket1 = 0, bucket2 = 0;
+    std::vector<int64_t> gmax;                           /* longest chain of each fold group */
+    mpgfx_mapping map1;
+    int bucket1 = 0;
   };
   std::vector<cls> classes;
+  /* phase 2: fold units of all classes, longest first, and the class table */
+  std::vector<mpgfx::bt_unit> units;
+  void *d_units = nullptr, *d_classes = nullptr, *d_counter = nullptr;
+  mpgfx_mapping map2;
+  int bucket_units = 0, bucket_len = 0;
   std::vector<hipStream_t> side;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -123,8 +131,13 @@ int launch_class (mpgfx_batch::cls &c, int cus, hipStream_t s)
 {
   int rc = c.ops->batch_launch (0, c.cplx, c.dev, 1, 64, s);
   if (!rc && c.dev.nitems > 0) rc = c.ops->batch_launch (1, c.cplx, c.dev, grid_of (c.map1, cus), c.map1.block, s);
-  if (!rc && c.dev.ngroups > 0) rc = c.ops->batch_launch (2, c.cplx, c.dev, grid_of (c.map2, cus), c.map2.block, s);
   return rc;
+}
+
+int launch_folds (mpgfx_batch *b, const mpgfx_mapping &mp, hipStream_t s)
+{
+  return batch_folds_launch ((const bt_class *) b->d_classes, (const bt_unit *) b->d_units, (i64) b->units.size (),
+                             (unsigned *) b->d_counter, grid_of (mp, b->cus), mp.block, s);
 }
 
 /* ---- tuning cache (flat JSON object: "key": [chunk, block, wg_per_cu]) ---- */
@@ -233,30 +246,49 @@ int tune_class (mpgfx_batch::cls &c, int cus, hipStream_t s, std::map<std::strin
       int rc = upload_items (c, c.map1.chunk, s);
       if (rc) return rc;
     }
-  /* phase 2 (needs the products of phase 1 in the scratch: run it once) */
-  std::string k2 = key_of (2, c, c.bucket2);
-  if (c.dev.ngroups > 0)
+  return MPGFX_OK;
+}
+
+/* phase 2: the all-class fold kernel's mapping (needs every class's products
+ * in the scratch: run phase 1 once first) */
+int tune_folds (mpgfx_batch *b, hipStream_t s, std::map<std::string, mpgfx_mapping> &cache, bool *dirty)
+{
+  if (b->units.empty ()) return MPGFX_OK;
+  char kb[96];
+  snprintf (kb, sizeof kb, "p2_all_U%d_L%d", b->bucket_units, b->bucket_len);
+  auto it = cache.find (kb);
+  if (it != cache.end ()) { b->map2 = it->second; return MPGFX_OK; }
+  for (auto &c : b->classes)
     {
-      auto it = cache.find (k2);
-      if (it != cache.end ()) c.map2 = it->second;
-      else
-        {
-          if (c.dev.nitems > 0 && time_phase (c, 1, c.map1, cus, s, 1) < 0.f) return MPGFX_EHIP;
-          const int blocks[3] = {64, 128, 256}, wgs[4] = {1, 2, 4, 8};
-          float best = -1.f;
-          mpgfx_mapping bm = DEFAULT_P2;
-          for (int bl : blocks)
-            for (int wg : wgs)
-              {
-                mpgfx_mapping mm = {0, bl, wg, 0};
-                float ms = time_phase (c, 2, mm, cus, s, 1);
-                if (ms > 0.f && (best < 0.f || ms < best)) { best = ms; bm = mm; }
-              }
-          c.map2 = bm;
-          cache[k2] = bm;
-          *dirty = true;
-        }
+      int rc = launch_class (c, b->cus, s);
+      if (rc) return rc;
     }
+  hipEvent_t e0, e1;
+  if (hipEventCreate (&e0) != hipSuccess) return MPGFX_EHIP;
+  if (hipEventCreate (&e1) != hipSuccess) { (void) hipEventDestroy (e0); return MPGFX_EHIP; }
+  const int blocks[3] = {64, 128, 256}, wgs[4] = {1, 2, 4, 8};
+  float best = -1.f;
+  mpgfx_mapping bm = DEFAULT_P2;
+  int rc = MPGFX_OK;
+  for (int bl : blocks)
+    for (int wg : wgs)
+      {
+        mpgfx_mapping mm = {0, bl, wg, 0};
+        float ms = 0.f;
+        rc = hip_ok ("tune", hipEventRecord (e0, s));
+        if (!rc) rc = launch_folds (b, mm, s);
+        if (!rc) rc = hip_ok ("tune", hipEventRecord (e1, s));
+        if (!rc) rc = hip_ok ("tune", hipEventSynchronize (e1));
+        if (!rc) rc = hip_ok ("tune", hipEventElapsedTime (&ms, e0, e1));
+        if (rc) break;
+        if (best < 0.f || ms < best) { best = ms; bm = mm; }
+      }
+  (void) hipEventDestroy (e0);
+  (void) hipEventDestroy (e1);
+  if (rc) return rc;
+  b->map2 = bm;
+  cache[kb] = bm;
+  *dirty = true;
   return MPGFX_OK;
 }
 
@@ -283,6 +315,9 @@ extern "C" int mpgfx_batch_destroy (mpgfx_batch *b)
   if (b->exec) (void) hipGraphExecDestroy (b->exec);
   if (b->graph) (void) hipGraphDestroy (b->graph);
   for (auto &c : b->classes) free_class (c);
+  void *ps[] = {b->d_units, b->d_classes, b->d_counter};
+  for (void *p : ps)
+    if (p) (void) hipFree (p);
   for (auto s : b->side) (void) hipStreamDestroy (s);
   delete b;
   return MPGFX_OK;
@@ -315,7 +350,7 @@ extern "C" int mpgfx_batch_create (const mpgfx_task *tasks, int64_t ntasks, int 
           idx[key] = (int) b->classes.size ();
           mpgfx_batch::cls c;
           c.pb = t.pb; c.cplx = t.cplx ? 1 : 0; c.ops = find_ops (t.pb);
-          c.map1 = DEFAULT_P1; c.map2 = DEFAULT_P2;
+          c.map1 = DEFAULT_P1;
           std::memset (&c.dev, 0, sizeof c.dev);
           b->classes.push_back (c);
         }
@@ -355,11 +390,13 @@ extern "C" int mpgfx_batch_create (const mpgfx_task *tasks, int64_t ntasks, int 
       int64_t ngroups = (c.nchains + R - 1) / R;
       std::vector<int64_t> gbase (ngroups > 0 ? ngroups : 1, 0);
       int64_t acc = 0;
+      c.gmax.assign ((size_t) ngroups, 0);
       for (int64_t g = 0; g < ngroups; g++)
         {
           gbase[g] = acc;
-          acc += R * ch[g * R].len;                       /* longest chain of the group */
-          c.total_chain_steps += ch[g * R].len;
+          c.gmax[g] = ch[g * R].len;                      /* longest chain of the group */
+          acc += R * c.gmax[g];
+          c.total_chain_steps += c.gmax[g];
         }
       while ((int64_t) ch.size () < ngroups * R) ch.push_back ({-1, 0, 0});
       const int64_t scr_n = acc;                          /* records per part */
@@ -369,7 +406,6 @@ extern "C" int mpgfx_batch_create (const mpgfx_task *tasks, int64_t ntasks, int 
                         [] (const std::pair<int, int64_t> &a, const std::pair<int, int64_t> &b2) { return a.second > b2.second; });
       double mean_task = c.axpy_like.empty () ? 0.0 : (double) c.total_values / (double) c.axpy_like.size ();
       c.bucket1 = bucket_of (mean_task);
-      c.bucket2 = bucket_of (c.nchains ? (double) c.total_chain_steps * (double) R / (double) c.nchains : 0.0);
       /* device allocations */
       size_t tb = c.tasks.size () * sizeof (bt_task);
       if (!rc) rc = hip_ok ("batch tasks", hipMalloc (&c.d_tasks, tb));
@@ -394,9 +430,52 @@ extern "C" int mpgfx_batch_create (const mpgfx_task *tasks, int64_t ntasks, int 
       c.dev.scr = (u64 *) c.d_scr_l;
       c.dev.scr_n = scr_n;
       c.dev.counters = (unsigned *) c.d_counters;
+      c.dev.pb = c.pb;
+      c.dev.cplx = c.cplx;
       if (!rc) rc = upload_items (c, c.map1.chunk, s);
       if (rc) break;
     }
+  /* ---- phase-2 units of all classes, longest first ---- */
+  b->map2 = DEFAULT_P2;
+  if (!rc)
+    {
+      std::vector<std::pair<int64_t, bt_unit>> lu;
+      int64_t maxlen = 0;
+      for (int ci = 0; ci < (int) b->classes.size (); ci++)
+        {
+          const auto &c = b->classes[ci];
+          for (int64_t g = 0; g < (int64_t) c.gmax.size (); g++)
+            for (int part = 0; part <= c.cplx; part++)
+              {
+                bt_unit u;
+                u.cls = ci; u.part = part; u.fg = g;
+                lu.push_back ({c.gmax[g], u});
+                maxlen = std::max (maxlen, c.gmax[g]);
+              }
+        }
+      std::stable_sort (lu.begin (), lu.end (),
+                        [] (const std::pair<int64_t, bt_unit> &x, const std::pair<int64_t, bt_unit> &y) { return x.first > y.first; });
+      for (auto &e : lu) b->units.push_back (e.second);
+      b->bucket_units = bucket_of ((double) b->units.size ());
+      b->bucket_len = bucket_of ((double) maxlen);
+      rc = hip_ok ("batch counter", hipMalloc (&b->d_counter, 64));
+      if (!rc && !b->units.empty ())
+        {
+          rc = hip_ok ("batch units", hipMalloc (&b->d_units, b->units.size () * sizeof (bt_unit)));
+          if (!rc) rc = hip_ok ("batch units copy", hipMemcpy (b->d_units, b->units.data (), b->units.size () * sizeof (bt_unit),
+                                                               hipMemcpyHostToDevice));
+        }
+      if (!rc) rc = hip_ok ("batch classes", hipMalloc (&b->d_classes, std::max<size_t> (1, b->classes.size ()) * sizeof (bt_class)));
+    }
+  /* the class table for the fold kernel (after tuning: items may be re-made) */
+  auto upload_classes = [&] () -> int {
+    std::vector<bt_class> tab;
+    for (auto &c : b->classes) tab.push_back (c.dev);
+    if (tab.empty ()) return (int) MPGFX_OK;
+    return hip_ok ("batch classes copy", hipMemcpy (b->d_classes, tab.data (), tab.size () * sizeof (bt_class),
+                                                    hipMemcpyHostToDevice));
+  };
+  if (!rc) rc = upload_classes ();
   /* ---- tuning ---- */
   if (!rc && tuning)
     {
@@ -404,9 +483,12 @@ extern "C" int mpgfx_batch_create (const mpgfx_task *tasks, int64_t ntasks, int 
       bool dirty = false;
       for (auto &c : b->classes)
         if (!rc) rc = tune_class (c, b->cus, s, cmap, &dirty);
+      if (!rc) rc = upload_classes ();
+      if (!rc) rc = tune_folds (b, s, cmap, &dirty);
       if (!rc && dirty) write_cache (cache, cmap);
     }
-  /* ---- graph: per class [reset -> products -> folds] on its own branch,
+  /* ---- graph: per class [reset -> products] on its own branch, then the
+   * folds of all classes in one kernel; 
    * captured on a private stream (the caller's may be the legacy null stream,
    * which cannot be captured); the graph is launched on the caller's stream ---- */
   if (!rc) rc = hip_ok ("batch stream sync", hipStreamSynchronize (s));
@@ -435,6 +517,7 @@ extern "C" int mpgfx_batch_create (const mpgfx_task *tasks, int64_t ntasks, int 
               if (!lrc) lrc = hip_ok ("batch join", hipEventRecord (jv[i], b->side[i]));
               if (!lrc) lrc = hip_ok ("batch join wait", hipStreamWaitEvent (cs, jv[i], 0));
             }
+          if (!lrc) lrc = launch_folds (b, b->map2, cs);
           hipGraph_t g = nullptr;
           int erc = hip_ok ("batch end capture", hipStreamEndCapture (cs, &g));
           rc = lrc ? lrc : erc;

@@ -1,6 +1,6 @@
-/* batch_pb.h -- per-precision persistent kernels of the batched execution (see
- * batch.h).  Included by kernels_pb.hip inside its anonymous namespace (PB, NL
- * defined there). */
+/* batch_pb.h -- per-precision kernels of the batched execution (see batch.h):
+ * the counter reset and the phase-1 products.  Included by kernels_pb.hip
+ * inside its anonymous namespace (PB, NL, COOP_R defined there). */
 
 /* record of step s of the chain at position p (records of NL+1 words) */
 __device__ __forceinline__ i64 bt_scr_index (const bt_class &c, i64 p, i64 s)
@@ -62,40 +62,8 @@ __global__ __launch_bounds__ (256) void k_batch_products (bt_class c)
                   p = bt_prod<CPLX> (t.in0l, t.in0s, t.m * t.n, g, t.in1l, t.in1s, t.n, k, part);
                   q = bt_scr_index (c, t.p0 + i, k);
                 }
-              st_rec (p, c.scr + ((i64) part * c.scr_n + q) * (NL + 1));
+              st_rec<PB> (p, c.scr + ((i64) part * c.scr_n + q) * (NL + 1));
             }
-        }
-    }
-}
-
-/* phase 2: a wavefront pulls a fold group (R chains, or for complex classes
- * the real or the imaginary parts of R chains: complex addition is
- * componentwise, so the two parts fold independently) and folds it
- * cooperatively, COOP_G lanes per chain */
-template <int CPLX>
-__global__ __launch_bounds__ (256) void k_batch_folds (bt_class c)
-{
-  const int lane = threadIdx.x & 63;
-  const i64 units = c.ngroups << CPLX;
-  for (;;)
-    {
-      unsigned w = 0;
-      if (lane == 0) w = atomicAdd (&c.counters[1], 1u);
-      w = __shfl (w, 0);
-      if ((i64) w >= units) break;
-      const i64 fg = (i64) w >> CPLX;
-      const int part = CPLX ? (int) (w & 1) : 0;
-      const int r = lane / COOP_G;
-      const bt_chain ch = c.chains[fg * COOP_R + r];
-      const i64 max_len = c.chains[fg * COOP_R].len;            /* longest first */
-      const i64 len = ch.task >= 0 ? ch.len : 0;
-      const cval acc = coop_fold (c.scr + (i64) part * c.scr_n * (NL + 1), c.gbase[fg] + r, COOP_R, len, max_len);
-      if (ch.task >= 0)
-        {
-          const bt_task &t = c.tasks[ch.task];
-          const i64 nout = (t.kind == MPGFX_TASK_DOT) ? 1 : t.m;
-          const i64 idx = (t.kind == MPGFX_TASK_DOT) ? 0 : ch.row;
-          coop_store (acc, t.outl + (i64) part * NL * nout, t.outs + (i64) part * nout, nout, idx);
         }
     }
 }
@@ -115,9 +83,6 @@ int batch_launch (int phase, int cplx, const bt_class &c, int grid, int block, h
       else hipLaunchKernelGGL ((k_batch_products<0>), dim3 (grid), dim3 (block), 0, s, c);
     }
   else
-    {
-      if (cplx) hipLaunchKernelGGL ((k_batch_folds<1>), dim3 (grid), dim3 (block), 0, s, c);
-      else hipLaunchKernelGGL ((k_batch_folds<0>), dim3 (grid), dim3 (block), 0, s, c);
-    }
+    return MPGFX_EINVAL;            /* the folds of all classes run in one kernel (batch_folds.hip) */
   return launch_status ("batch");
 }

@@ -28,13 +28,9 @@
  *     a signed shift, applied as a two-lane funnel shift;
  *   - RNDN on the kept PB bits (round bit, sticky, ties to even) and the
  *     increment, again with a propagate mask.
- * Included by kernels_pb.hip inside its anonymous namespace (PB, NL defined
- * there). */
-
-constexpr int COOP_G = coop_lanes (PB);
-constexpr int COOP_R = 64 / COOP_G;
-constexpr int COOP_SB = cu_freal<PB>::SB;
-static_assert (NL + 2 <= 64, "cooperative fold needs N + 2 <= 64 limbs");
+ * Everything is templated on the precision; the header is included inside
+ * an anonymous namespace of each translation unit that folds (the
+ * per-precision kernels, and the batch's all-precision fold kernel). */
 
 /* one lane's part of a number in the frame */
 struct cval
@@ -52,10 +48,11 @@ __device__ __forceinline__ u64 bperm64 (u64 v, int src)
   return ((u64) hi << 32) | lo;
 }
 
-/* the group's bits of a wave mask, shifted down to bit 0 */
+/* the group's bits of a wave mask, shifted down to bit 0 (groups of G lanes) */
+template <int G>
 __device__ __forceinline__ u64 gbits (u64 mask, int base)
 {
-  const u64 gm = (COOP_G == 64) ? ~0ull : ((1ull << (COOP_G & 63)) - 1);
+  const u64 gm = (G == 64) ? ~0ull : ((1ull << (G & 63)) - 1);
   return (mask >> base) & gm;
 }
 
@@ -68,14 +65,18 @@ __device__ __forceinline__ cval coop_zero ()
   return z;
 }
 
-/* RNDN(a + b) with cu_fadd's semantics; every lane of the wavefront must call it */
+/* RNDN(a + b) with cu_fadd's semantics at precision PB; every lane of the
+ * wavefront must call it */
+template <int PB>
 __device__ __forceinline__ cval coop_add (const cval &a, const cval &b)
 {
+  constexpr int NL = (PB + 63) / 64, COOP_G = coop_lanes (PB), COOP_SB = 64 * NL - PB;
+  static_assert (NL + 2 <= COOP_G && COOP_G <= 64, "frame does not fit the lane group");
   const int lane = (int) (threadIdx.x & 63), j = lane & (COOP_G - 1), base = lane & ~(COOP_G - 1);
   const bool za = a.exp == CU_FREAL_ZERO_EXP, zb = b.exp == CU_FREAL_ZERO_EXP;
 
   /* order by magnitude (zero is the smallest) */
-  const u64 neq = gbits (__ballot (a.m != b.m), base), gt = gbits (__ballot (a.m > b.m), base);
+  const u64 neq = gbits<COOP_G> (__ballot (a.m != b.m), base), gt = gbits<COOP_G> (__ballot (a.m > b.m), base);
   const int top_d = 63 - __clzll (neq | 1ull);
   const bool lge = (neq == 0) || ((gt >> top_d) & 1);
   const bool age = zb || (!za && (a.exp > b.exp || (a.exp == b.exp && lge)));
@@ -95,7 +96,7 @@ __device__ __forceinline__ cval coop_add (const cval &a, const cval &b)
   b1 = s0 + 1 < COOP_G ? b1 : 0;
   const u64 T = bs ? ((b0 >> bs) | (b1 << (64 - bs))) : b0;
   const bool lost = (j < ls) ? (B != 0) : (j == ls && bs != 0 && (B << (64 - bs)) != 0);
-  bool st = gbits (__ballot (lost), base) != 0;
+  bool st = gbits<COOP_G> (__ballot (lost), base) != 0;
 
   /* D = A + T, or A - T - st (sticky borrow), over lanes 0..N */
   const bool sub = (sA != sB) && !zB;
@@ -108,7 +109,7 @@ __device__ __forceinline__ cval coop_add (const cval &a, const cval &b)
   res = (sub && j > NL) ? 0ull : res;
 
   /* normalise: leading one to bit 63 of lane N */
-  const u64 nz = gbits (__ballot (res != 0), base);
+  const u64 nz = gbits<COOP_G> (__ballot (res != 0), base);
   const bool zres = nz == 0;
   const int top = 63 - __clzll (nz | 1ull);
   const int lzl = __clzll (res | 1ull);                       /* this lane's leading zeros */
@@ -122,7 +123,7 @@ __device__ __forceinline__ cval coop_add (const cval &a, const cval &b)
   w1 = (c1 >= 0 && c1 < COOP_G) ? w1 : 0ull;
   u64 Rn = r ? ((w0 >> r) | (w1 << (64 - r))) : w0;
   /* a right shift (carry-out) drops bit 0 of the guard limb into the sticky bit */
-  st = st || (off > 0 && gbits (__ballot (j == 0 && (res & 1ull)), base) != 0);
+  st = st || (off > 0 && gbits<COOP_G> (__ballot (j == 0 && (res & 1ull)), base) != 0);
   i64 eR = eA - L;
 
   /* RNDN on the kept bits: lanes 1..N minus the low SB bits of lane 1 */
@@ -139,9 +140,9 @@ __device__ __forceinline__ cval coop_add (const cval &a, const cval &b)
       sf = ((j == 1) && (Rn & ((1ull << ((COOP_SB - 1) & 63)) - 1)) != 0) || ((j == 0) && Rn != 0);
       lf = (j == 1) && ((Rn >> (COOP_SB & 63)) & 1ull);
     }
-  const bool rb = gbits (__ballot (rf), base) != 0;
-  const bool sb = gbits (__ballot (sf), base) != 0;
-  const bool lb = gbits (__ballot (lf), base) != 0;
+  const bool rb = gbits<COOP_G> (__ballot (rf), base) != 0;
+  const bool sb = gbits<COOP_G> (__ballot (sf), base) != 0;
+  const bool lb = gbits<COOP_G> (__ballot (lf), base) != 0;
   const bool inc = rb && (st || sb || lb);
   const u64 low = (COOP_SB == 0) ? 0ull : ((1ull << (COOP_SB & 63)) - 1);
   const u64 kept = (j == 1) ? (Rn | low) : Rn;
@@ -165,8 +166,10 @@ __device__ __forceinline__ cval coop_add (const cval &a, const cval &b)
 
 /* frame part of a value stored as a record of N limbs then the sign/exponent
  * word (NL + 1 words); lanes 1..N hold limbs 0..N-1 */
+template <int PB>
 __device__ __forceinline__ cval coop_load_rec (const u64 *__restrict__ rec)
 {
+  constexpr int NL = (PB + 63) / 64, COOP_G = coop_lanes (PB);
   const int j = (int) (threadIdx.x & (COOP_G - 1));
   cval v;
   v.m = (j >= 1 && j <= NL) ? rec[j - 1] : 0ull;
@@ -177,16 +180,20 @@ __device__ __forceinline__ cval coop_load_rec (const u64 *__restrict__ rec)
 }
 
 /* store the group's number to an MP array (limb planes + sign/exponent word) */
+template <int PB>
 __device__ __forceinline__ void coop_store (const cval &v, u64 *__restrict__ l, i64 *__restrict__ se, i64 n, i64 i)
 {
+  constexpr int NL = (PB + 63) / 64, COOP_G = coop_lanes (PB);
   const int j = (int) (threadIdx.x & (COOP_G - 1));
   if (j >= 1 && j <= NL) l[(i64) (j - 1) * n + i] = v.m;
   if (j == 0) se[i] = (i64) ((u64) v.exp << 1) | (v.sign < 0 ? 1 : 0);
 }
 
 /* record store of a whole value (one lane) */
+template <int PB>
 __device__ __forceinline__ void st_rec (const cu_freal<PB> &v, u64 *__restrict__ rec)
 {
+  constexpr int NL = (PB + 63) / 64;
 #pragma unroll
   for (int k = 0; k < NL; k++) rec[k] = v.m[k];
   rec[NL] = (u64) (((i64) v.exp << 1) | (v.sign < 0 ? 1 : 0));
@@ -198,13 +205,15 @@ __device__ __forceinline__ void st_rec (const cu_freal<PB> &v, u64 *__restrict__
  * keeps its value.  Records are prefetched COOP_PD steps ahead. */
 constexpr int COOP_PD = 4;
 
+template <int PB>
 __device__ __forceinline__ cval coop_fold (const u64 *__restrict__ recs, i64 rec0, i64 step, i64 len, i64 max_len)
 {
+  constexpr int NL = (PB + 63) / 64;
   cval acc = coop_zero ();
   cval ring[COOP_PD];
 #pragma unroll
   for (int d = 0; d < COOP_PD; d++)
-    ring[d] = (d < len) ? coop_load_rec (recs + (rec0 + step * d) * (NL + 1)) : coop_zero ();
+    ring[d] = (d < len) ? coop_load_rec<PB> (recs + (rec0 + step * d) * (NL + 1)) : coop_zero ();
   for (i64 s0 = 0; s0 < max_len; s0 += COOP_PD)
     {
 #pragma unroll
@@ -213,12 +222,12 @@ __device__ __forceinline__ cval coop_fold (const u64 *__restrict__ recs, i64 rec
           const i64 s = s0 + d;
           if (s < max_len)
             {
-              const cval nv = coop_add (acc, ring[d]);
+              const cval nv = coop_add<PB> (acc, ring[d]);
               const bool live = s < len;
               acc.m = live ? nv.m : acc.m;
               acc.exp = live ? nv.exp : acc.exp;
               acc.sign = live ? nv.sign : acc.sign;
-              if (s + COOP_PD < len) ring[d] = coop_load_rec (recs + (rec0 + step * (s + COOP_PD)) * (NL + 1));
+              if (s + COOP_PD < len) ring[d] = coop_load_rec<PB> (recs + (rec0 + step * (s + COOP_PD)) * (NL + 1));
             }
         }
     }

@@ -32,7 +32,7 @@ struct pb_ops
   int (*exact_fallback) (i64 m, i64 n, i64 k, i64 r0, i64 rows, const u64 *Al, const i64 *Ase, const u64 *Bl,
                          const i64 *Bse, const i64 *eminA, const i64 *eminB, int nw, i64 *wsp, u64 *Cl, i64 *Cse,
                          hipStream_t);
-  /* batched persistent kernels (batch_pb.h): phase 0 reset, 1 products, 2 folds */
+  /* batched persistent kernels (batch_pb.h): phase 0 reset, 1 products */
   int (*batch_launch) (int phase, int cplx, const bt_class &c, int grid, int block, hipStream_t);
 };
 
@@ -42,6 +42,11 @@ MPGFX_PB_LIST (MPGFX_DECL)
 
 /* table lookup; NULL if pb is not compiled in */
 const pb_ops *find_ops (int pb);
+
+/* phase 2 of the batched execution: the folds of all classes (batch_folds.hip);
+ * resets `counter` first */
+int batch_folds_launch (const bt_class *cls, const bt_unit *units, i64 nunits, unsigned *counter, int grid, int block,
+                        hipStream_t s);
 
 /* exact-dot GEMM (exact_gemm.hip) */
 size_t exact_workspace_bound (int pb, i64 m, i64 n, i64 k);

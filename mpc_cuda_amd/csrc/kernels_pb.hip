@@ -167,6 +167,8 @@ __global__ __launch_bounds__ (TPB) void k_gemm_cplx (i64 m, i64 n, i64 k, const 
 }
 
 #include "coop.h"
+constexpr int COOP_G = coop_lanes (PB);
+constexpr int COOP_R = 64 / COOP_G;
 
 /* GEMV, per-op, in two phases.  Phase 1: every product P[i,k] = RNDN(A[i,k] *
  * x[k]) in parallel (lane per product), into records of N limbs + the
@@ -185,7 +187,7 @@ __global__ __launch_bounds__ (TPB) void k_gemv_products (i64 m, i64 k, i64 r0, i
     {
       const i64 fg = e / kr, rem = e - fg * kr, kk = rem / COOP_R, i = fg * COOP_R + (rem - kk * COOP_R);
       if (i < mc)
-        st_rec (cu_fp::cu_fmul<PB> (ld<PB> (Al, Ase, nA, (r0 + i) * k + kk), ldc<PB> (xl, xse, k, kk)),
+        st_rec<PB> (cu_fp::cu_fmul<PB> (ld<PB> (Al, Ase, nA, (r0 + i) * k + kk), ldc<PB> (xl, xse, k, kk)),
                 P + e * (NL + 1));
     }
 }
@@ -196,8 +198,8 @@ __global__ __launch_bounds__ (64) void k_gemv_fold (i64 m, i64 k, i64 r0, i64 mc
   const i64 fg = blockIdx.x;
   const int r = (int) (threadIdx.x / COOP_G);
   const i64 i = fg * COOP_R + r;
-  const cval acc = coop_fold (P, fg * k * COOP_R + r, COOP_R, i < mc ? k : 0, k);
-  if (i < mc) coop_store (acc, yl, yse, m, r0 + i);
+  const cval acc = coop_fold<PB> (P, fg * k * COOP_R + r, COOP_R, i < mc ? k : 0, k);
+  if (i < mc) coop_store<PB> (acc, yl, yse, m, r0 + i);
 }
 
 /* AoS (reference struct layout) <-> SoA, one lane per real value */
