@@ -108,9 +108,11 @@ CUFP_INL int clz32 (u32 x)
 /* ---------------------------------------------------------------------------
  * Exact product p[2W] = a[W] * b[W] of two W-word significands.
  *
- * Small W: schoolbook by rows on 32-bit words (one 32x32+64 multiply-add per
- * word product plus the carry add).
- * W >= 8: the operands are re-cut into LB-bit limbs (LB = 29, or 28 for the
+ * W < 16 (pb < 512): schoolbook by rows on 32-bit words (one 32x32+64
+ * multiply-add per word product plus the carry add); at these sizes the limb
+ * split and recombination below cost more than they save (measured on the
+ * pb 256 GEMM and AXPY).
+ * W >= 16: the operands are re-cut into LB-bit limbs (LB = 29, or 28 for the
  * widest precisions) and the product is accumulated by columns: a column of
  * up to 63 (255) limb products plus the incoming carry cannot overflow 64 bits,
  * so every limb product is exactly one v_mad_u64_u32 with the running column
@@ -194,7 +196,7 @@ CUFP_INL void mul_words_limbs (u32 (&p)[2 * W], const u32 (&a)[W], const u32 (&b
 template <int W>
 CUFP_INL void mul_words (u32 (&p)[2 * W], const u32 (&a)[W], const u32 (&b)[W])
 {
-  if constexpr (W >= 8)
+  if constexpr (W >= 16)
     mul_words_limbs<W> (p, a, b);
   else
     mul_words_rows<W> (p, a, b);
