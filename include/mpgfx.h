@@ -158,6 +158,28 @@ int mpgfx_pack (int pb, int complex, int64_t n, const void *aos,
 int mpgfx_unpack (int pb, int complex, int64_t n, const uint64_t *limbs,
                   const int64_t *se, void *aos, void *stream);
 
+/* Interop with host arrays of MPFR numbers (the on-wire format either side of
+ * the path).  `x` points to n MPFR value structs (`__mpfr_struct`: prec,
+ * sign, exp, limb pointer -- the layout MPFR's custom interface macros read,
+ * mpfr.h) placed `stride` bytes apart: sizeof(mpfr_t) for an mpfr_t array;
+ * for an mpc_t array use stride sizeof(mpc_t) with x = &z[0].re (real parts)
+ * and x = &z[0].im (imaginary parts).  MPFR's significand limbs are the
+ * storage of cu_freal<PB> (left-justified, little-endian limbs, cu_freal.cuh:12-15)
+ * and its exponent is cu_freal's `exp`, so each value copies as-is; nothing
+ * is rounded.
+ * import: every value must have precision pb and be zero or regular (NaN and
+ * infinities give MPGFX_EINVAL); the limbs land in the device MP array.
+ * export: every struct must have precision pb and a limb buffer of
+ * ceil(pb/64) limbs (as mpfr_init2 leaves it); limbs, sign and exponent are
+ * written (a zero keeps the sign of its sign field).
+ * Both stage through host memory and synchronise `stream`.  No counterpart in
+ * the reference (its host code converts through doubles or mpz,
+ * tools/test_freal.cu:29-40). */
+int mpgfx_import_mpfr (int pb, int64_t n, const void *x, size_t stride,
+                       uint64_t *limbs, int64_t *se, void *stream);
+int mpgfx_export_mpfr (int pb, int64_t n, const uint64_t *limbs, const int64_t *se,
+                       void *x, size_t stride, void *stream);
+
 /* n seeded synthetic nonzero values (value i = start + i of the stream keyed by
  * seed: significand limbs random with the top bit set and the low 64N-pb bits
  * clear, random sign, exponent uniform in [emin, emax]) -- the generator of the
