@@ -193,6 +193,61 @@ CUFP_INL void mul_words_limbs (u32 (&p)[2 * W], const u32 (&a)[W], const u32 (&b
   });
 }
 
+/* ---------------------------------------------------------------------------
+ * Short product: the top H words of a[W] * b[W] (H <= 2W - 3) from the limb
+ * columns at and above K0 only, the column that holds product bit LSB =
+ * 32 (2W - H), the lowest bit kept.  The columns below K0 each hold less than
+ * (k+1) 2^(2 LB), so together they are worth less than
+ * K0 2^(2 LB) 2^(LB (K0-1)) / (1 - 2^-LB) < 2^(LB K0 + LB + lg K0 + 1); leaving
+ * out their carries and cutting the sum at bit LSB makes the result p a lower
+ * bound of the true top words with an error below 2^ERR units of p's last
+ * word.  When both operands have their top bit set the top word of p keeps
+ * bit 31 or bit 30 set: the limbs at and above K0 of a alone, times b, are at
+ * least 2^(32W-1) 2^(32W-1).
+ * ------------------------------------------------------------------------- */
+template <int W, int H>
+struct short_cut
+{
+  typedef limb_cut<W> C;
+  static_assert (H <= 2 * W - 3, "short product wider than the product");
+  static constexpr int LSB = 32 * (2 * W - H);
+  static constexpr int K0 = LSB / C::LB;
+  static constexpr int lg (int x) { return x <= 1 ? 0 : 1 + lg ((x + 1) / 2); }
+  static constexpr int ERR = C::LB * K0 + C::LB + lg (K0) + 2 - LSB;
+};
+
+template <int W, int H>
+CUFP_INL void mul_words_short (u32 (&p)[H], const u32 (&a)[W], const u32 (&b)[W])
+{
+  typedef limb_cut<W> C;
+  constexpr int NL = C::NL, LB = C::LB, K0 = short_cut<W, H>::K0;
+  u32 La[NL], Lb[NL], P[2 * NL];
+  words_to_limbs<W> (La, a);
+  words_to_limbs<W> (Lb, b);
+  u64 acc = 0;
+  sfor<K0, 2 * NL - 1> ([&] (auto K) {
+    constexpr int k = decltype (K)::value;
+    constexpr int i0 = (k >= NL) ? k - NL + 1 : 0, i1 = (k < NL) ? k + 1 : NL;
+    sfor<i0, i1> ([&] (auto I) {
+      constexpr int i = decltype (I)::value;
+      acc = (u64) La[i] * Lb[k - i] + acc;
+    });
+    P[k] = (u32) acc & C::MASK;
+    acc >>= LB;
+  });
+  P[2 * NL - 1] = (u32) acc;
+  sfor<0, H> ([&] (auto Q) {
+    constexpr int q = 2 * W - H + decltype (Q)::value;
+    constexpr int k0 = (32 * q) / LB, s0 = 32 * q - LB * k0;
+    static_assert (k0 >= K0, "word below the computed columns");
+    const u64 l0 = P[k0];
+    const u64 l1 = (k0 + 1 < 2 * NL) ? P[k0 + 1] : 0u;
+    const u64 l2 = (k0 + 2 < 2 * NL) ? P[k0 + 2] : 0u;
+    const u64 v = l0 | (l1 << LB) | (l2 << (2 * LB));
+    p[decltype (Q)::value] = (u32) (v >> s0);
+  });
+}
+
 template <int W>
 CUFP_INL void mul_words (u32 (&p)[2 * W], const u32 (&a)[W], const u32 (&b)[W])
 {
@@ -256,6 +311,7 @@ struct rres
   int sign;
   i64 exp;
   bool zero;
+  bool amb;          /* ERRB > 0: the rounding is not decided by the bound */
   u32 k[KW];
 };
 
@@ -273,7 +329,13 @@ CUFP_INL bool mag_ge (i64 ea, const u32 (&A)[LW], i64 eb, const u32 (&B)[LW])
   return (ea != eb) ? (ea > eb) : (c >= 0);
 }
 
-template <int KW, int LW>
+/* ERRB > 0: the operands are lower or upper bounds of the true terms, off by
+ * less than 2^ERRB units of the last word of the larger one (together), and
+ * r.amb reports whether the true sum could round differently: that is when
+ * it lies within that error of a rounding midpoint, or the computed sum is 0.
+ * With 4 guard words below the kept ones (LW = KW + 3) the test reads the
+ * top 64 guard bits. */
+template <int KW, int LW, int ERRB = 0>
 CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
                              int sb, i64 eb, bool zb, const u32 (&Bi)[LW])
 {
@@ -386,6 +448,25 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
     {
       u32 lo = (i >= 1) ? E[i - 1] : 0u;
       E[i] = fshl (E[i], lo, bl);
+    }
+  r.amb = false;
+  if constexpr (ERRB > 0)
+    {
+      /* E's last bit is worth 2^(cy - lz - 32) units of A's last word, so
+       * the error is below 2^(ERRB + 32 - cy + lz) of E's last bits; the
+       * guard bits g = E[0..3] (round bit on top) are safe unless
+       * |g - 2^127| <= that, tested on the bits of E[3]:E[2] above it */
+      static_assert (L - 1 - KW == 4, "the test reads 4 guard words");
+      static_assert (ERRB >= 33, "error position below the tested guard bits");
+      const int t = ERRB + 32 - (int) cy + (int) lz - 64;
+      const u64 x = ((u64) E[3] << 32) | E[2];
+      bool amb = true;
+      if (t < 63)
+        {
+          const u64 m = 1ull << (63 - t), q = x >> t;
+          amb = (q == m) || (q + 1 == m);
+        }
+      r.amb = amb || zres;
     }
   u32 co = round_top<KW, L - 1> (r.k, E, 0u, st);
   r.sign = zres ? 1 : sA;

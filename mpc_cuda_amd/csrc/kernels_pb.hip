@@ -91,12 +91,12 @@ __global__ __launch_bounds__ (TPB) void k_axpy_real (i64 n, const u64 *__restric
 __global__ __launch_bounds__ (TPB) void k_axpy_cplx (i64 n, const u64 *__restrict__ al, const i64 *__restrict__ ase,
                                                      const u64 *__restrict__ xl, const i64 *__restrict__ xse,
                                                      const u64 *__restrict__ yl, const i64 *__restrict__ yse,
-                                                     u64 *__restrict__ zl, i64 *__restrict__ zse)
+                                                     u64 *__restrict__ zl, i64 *__restrict__ zse, i64 i0)
 {
   const i64 o = (i64) NL * n;
   const cu_freal<PB> ar = ldc<PB> (al, ase, 1, 0), ai = ldc<PB> (al + NL, ase + 1, 1, 0);
   const i64 stride = (i64) gridDim.x * blockDim.x;
-  for (i64 t = (i64) blockIdx.x * blockDim.x + threadIdx.x; t < 2 * n; t += stride)
+  for (i64 t = 2 * i0 + (i64) blockIdx.x * blockDim.x + threadIdx.x; t < 2 * n; t += stride)
     {
       const i64 i = t >> 1;
       const int p = (int) (t & 1);
@@ -105,6 +105,100 @@ __global__ __launch_bounds__ (TPB) void k_axpy_cplx (i64 n, const u64 *__restric
       cu_freal<PB> prod = cu_fp::core::fmm<PB> (ar, u, ai, v, p == 0);
       cu_freal<PB> y = ld<PB> (yl + p * o, yse + p * n, n, i);
       st<PB> (cu_fp::cu_fadd<PB> (prod, y), zl + p * o, zse + p * n, n, i);
+    }
+}
+
+/* The same complex AXPY with its operand traffic overlapped with arithmetic,
+ * for the precisions whose register footprint leaves one wavefront per SIMD
+ * (so no other wavefront hides a load).  Each wavefront loops over chunks of
+ * 32 elements (64 lanes, two per element as above) and, while it computes
+ * chunk c, copies chunk c + stride of x and y from HBM straight into LDS
+ * (global_load_lds_dwordx4, no registers held).  LDS holds two buffers per
+ * wavefront; a buffer is 4 NL + 4 rows of 256 bytes (32 values x 8 bytes):
+ * rows 0..2NL-1 the limbs of x (real then imaginary), 2NL..4NL-1 those of y,
+ * then the sign/exponent words of xr, xi, yr, yi.  Copy instruction q moves
+ * rows 4q..4q+3 (16 lanes per row, 16 bytes per lane).  Only whole chunks go
+ * through this kernel; the n mod 32 tail goes through k_axpy_cplx. */
+constexpr int AXC_WAVES = TPB / 64;
+constexpr int AXC_ROWS = 4 * NL + 4;
+constexpr int AXC_BUF = AXC_ROWS * 256;
+constexpr bool AXC_PIPE = PB >= 512 && 2 * AXC_WAVES * AXC_BUF <= 160 * 1024;
+typedef __attribute__ ((address_space (3))) void *lds_ptr;
+
+__device__ __forceinline__ cu_freal<PB> lds_val (const unsigned char *buf, int row, int srow, int e)
+{
+  cu_freal<PB> r;
+#pragma unroll
+  for (int k = 0; k < NL; k++) r.m[k] = *(const u64 *) (buf + (row + k) * 256 + e * 8);
+  const i64 w = *(const i64 *) (buf + srow * 256 + e * 8);
+  r.sign = (w & 1) ? -1 : 1;
+  r.exp = (long) (w >> 1);
+  return r;
+}
+
+__global__ __launch_bounds__ (TPB) void k_axpy_cplx_pipe (i64 nchunks, i64 n, const u64 *__restrict__ al,
+                                                          const i64 *__restrict__ ase, const u64 *__restrict__ xl,
+                                                          const i64 *__restrict__ xse, const u64 *__restrict__ yl,
+                                                          const i64 *__restrict__ yse, u64 *__restrict__ zl,
+                                                          i64 *__restrict__ zse)
+{
+  __shared__ __attribute__ ((aligned (16))) unsigned char lds[AXC_PIPE ? 2 * AXC_WAVES * AXC_BUF : 16];
+  if constexpr (AXC_PIPE)
+    {
+      const int L = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane (threadIdx.x >> 6);
+      const int e = L >> 1, p = L & 1, quarter = L >> 4, lane_e = 2 * (L & 15);
+      const i64 o = (i64) NL * n, wstride = (i64) gridDim.x * AXC_WAVES;
+      const cu_freal<PB> ar = ldc<PB> (al, ase, 1, 0), ai = ldc<PB> (al + NL, ase + 1, 1, 0);
+      unsigned char *const mine = lds + w * 2 * AXC_BUF;
+      i64 c = (i64) blockIdx.x * AXC_WAVES + w;
+      if (c >= nchunks) return;
+      auto copy_in = [&] (i64 cc, unsigned char *dst) {
+        const i64 eoff = cc * 32 + lane_e;
+#pragma unroll
+        for (int q = 0; q <= NL; q++)
+          {
+            const int r = 4 * q + quarter;
+            const void *src;
+            if (r < 2 * NL) src = xl + (i64) r * n + eoff;
+            else if (r < 4 * NL) src = yl + (i64) (r - 2 * NL) * n + eoff;
+            else
+              {
+                const int sr = r - 4 * NL;
+                src = (sr < 2 ? xse : yse) + (sr & 1) * n + eoff;
+              }
+            __builtin_amdgcn_global_load_lds (src, (lds_ptr) (dst + q * 1024), 16, 0, 0);
+          }
+      };
+      copy_in (c, mine);
+      __builtin_amdgcn_s_waitcnt (0);     /* LDS copies are not tracked by the compiler */
+      for (int j = 0;; j++)
+        {
+          const unsigned char *cur = mine + (j & 1) * AXC_BUF;
+          const i64 cn = c + wstride;
+          const cu_freal<PB> u = lds_val (cur, p ? NL : 0, 4 * NL + p, e);
+          const cu_freal<PB> v = lds_val (cur, p ? 0 : NL, 4 * NL + 1 - p, e);
+          if (cn < nchunks) copy_in (cn, mine + ((j + 1) & 1) * AXC_BUF);
+          bool amb;
+          cu_freal<PB> prod = cu_fp::core::fmm_short<PB> (ar, u, ai, v, p == 0, amb);
+          if (__builtin_amdgcn_ballot_w64 (amb) != 0)
+            {
+              /* exact products for the lanes the short ones leave open; the
+               * operands are read again from LDS rather than held */
+              const cu_freal<PB> u2 = lds_val (cur, p ? NL : 0, 4 * NL + p, e);
+              const cu_freal<PB> v2 = lds_val (cur, p ? 0 : NL, 4 * NL + 1 - p, e);
+              const cu_freal<PB> ex = cu_fp::core::fmm_exact<PB> (ar, u2, ai, v2, p == 0);
+              if (amb) prod = ex;
+            }
+          const cu_freal<PB> y = lds_val (cur, 2 * NL + p * NL, 4 * NL + 2 + p, e);
+          const cu_freal<PB> z = cu_fp::cu_fadd<PB> (prod, y);
+          /* the next chunk has landed long since; waiting here, before this
+           * chunk's stores, keeps the next iteration's LDS reads from waiting
+           * on them */
+          __builtin_amdgcn_s_waitcnt (0);
+          st<PB> (z, zl + p * o, zse + p * n, n, c * 32 + e);
+          if (cn >= nchunks) break;
+          c = cn;
+        }
     }
 }
 
@@ -258,7 +352,17 @@ int axpy_real (i64 n, const u64 *al, const i64 *ase, const u64 *xl, const i64 *x
 int axpy_cplx (i64 n, const u64 *al, const i64 *ase, const u64 *xl, const i64 *xse, const u64 *yl, const i64 *yse,
                u64 *ol, i64 *ose, hipStream_t s)
 {
-  hipLaunchKernelGGL (k_axpy_cplx, dim3 (grid_for (2 * n, TPB, 32)), dim3 (TPB), 0, s, n, al, ase, xl, xse, yl, yse, ol, ose);
+  i64 i0 = 0;
+  if (AXC_PIPE && n >= 32)
+    {
+      const i64 nchunks = n / 32, wg = (nchunks + AXC_WAVES - 1) / AXC_WAVES;
+      hipLaunchKernelGGL (k_axpy_cplx_pipe, dim3 ((unsigned) (wg < 256 ? wg : 256)), dim3 (TPB), 0, s, nchunks, n, al,
+                          ase, xl, xse, yl, yse, ol, ose);
+      i0 = nchunks * 32;
+    }
+  if (i0 < n)
+    hipLaunchKernelGGL (k_axpy_cplx, dim3 (grid_for (2 * (n - i0), TPB, 32)), dim3 (TPB), 0, s, n, al, ase, xl, xse,
+                        yl, yse, ol, ose, i0);
   return launch_status ("axpy_complex");
 }
 int gemm_perop (i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl, const i64 *Bse, u64 *Cl,

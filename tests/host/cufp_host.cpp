@@ -78,3 +78,33 @@ extern "C" double host_roundtrip_double (int pb, double d, int *exp_out)
 #undef C
   return 0.0;
 }
+
+/* one component of a complex product, RNDN(a*b -/+ c*d), by the short-product
+ * path (with its `amb` flag) and by the exact path, so tests can check that
+ * the short path's result is the exact one wherever it does not flag the lane.
+ * Operands: four real arrays stacked, limbs (4, N, n), se (4, n). */
+template <int PB> void fmm_chk (int64_t n, const uint64_t *l, const int64_t *se, int sub, uint64_t *sl,
+                                int64_t *sse, uint64_t *el, int64_t *ese, uint8_t *amb)
+{
+  const int64_t o = (int64_t) cu_freal<PB>::N * n;
+#pragma omp parallel for
+  for (int64_t i = 0; i < n; i++)
+    {
+      cu_freal<PB> a = ld<PB> (l, se, n, i), b = ld<PB> (l + o, se + n, n, i);
+      cu_freal<PB> c = ld<PB> (l + 2 * o, se + 2 * n, n, i), d = ld<PB> (l + 3 * o, se + 3 * n, n, i);
+      bool f = false;
+      cu_freal<PB> r = core::fmm_short<PB> (a, b, c, d, sub != 0, f);
+      st<PB> (r, sl, sse, n, i);
+      st<PB> (core::fmm_exact<PB> (a, b, c, d, sub != 0), el, ese, n, i);
+      amb[i] = f ? 1 : 0;
+    }
+}
+#define PB_SHORT(X) X(512) X(1024) X(1056) X(2048)
+extern "C" int host_fmm_check (int pb, int64_t n, const uint64_t *l, const int64_t *se, int sub, uint64_t *sl,
+                               int64_t *sse, uint64_t *el, int64_t *ese, uint8_t *amb)
+{
+#define C(P) if (pb == P) { fmm_chk<P> (n, l, se, sub, sl, sse, el, ese, amb); return 0; }
+  PB_SHORT (C)
+#undef C
+  return -1;
+}

@@ -92,6 +92,64 @@ def test_axpy_vs_golden_and_oracle(gpu, pb):
     assert same(M.axpy(ca.cuda(), cx.cuda(), cy.cuda()), O.axpy_complex(pb, ca.pair(), cx.pair(), cy.pair())) == 0
 
 
+@pytest.mark.parametrize("pb", [512, 1024, 1056])
+def test_axpy_complex_chunk_loop(gpu, pb):
+    """Wide precisions run the complex AXPY as a loop over 32-element chunks
+    per wavefront with the next chunk copied into LDS ahead of use: three
+    chunks per wavefront plus a 17-element tail, oracle bit-exact."""
+    n = 3 * 1024 * 32 + 17
+    ca = M.random_complex(pb, 1, 30 + pb, -16, 16)
+    cx = M.random_complex(pb, n, 31 + pb, -16, 16)
+    cy = M.random_complex(pb, n, 32 + pb, -16, 16)
+    assert same(M.axpy(ca.cuda(), cx.cuda(), cy.cuda()), O.axpy_complex(pb, ca.pair(), cx.pair(), cy.pair())) == 0
+
+
+def _cplx_from(re, im, pb):
+    return MPArray(pb, np.ascontiguousarray(np.stack([re.limbs, im.limbs])), np.ascontiguousarray(np.stack([re.se, im.se])),
+                   True)
+
+
+@pytest.mark.parametrize("pb", [512, 1024, 1056, 2048])
+def test_complex_products_the_short_path_leaves_open(gpu, pb):
+    """Complex products whose rounding the short products cannot decide, mixed
+    into random ones: a real part on a midpoint that only the low product bits
+    move off ((2^(pb-1)+1)^2 2^-2pb + 2^(-pb-2)), exact cancellation (+0) and
+    cancellation of all but a few bits.  Through the elementwise product and
+    the AXPY (inside whole 32-element chunks and in the tail), oracle
+    bit-exact."""
+    from mpcases import pack
+    from mpc_cuda_amd.arrays import nlimbs
+    top = 1 << (pb - 1)
+    v = lambda m, e, s=1: (s, e, m << (64 * nlimbs(pb) - pb), False)
+    zero = (1, -0x7FFFFFFFFFFFFF, 0, True)
+    n = 100
+    rnd = M.random_complex(pb, n, 900 + pb, -20, 20)
+    rr, ri = rnd.real_part(), rnd.imag_part()
+    from mpcases import unpack
+    Rr, Ri = unpack(rr), unpack(ri)
+    # elementwise: a*b per element
+    Ar, Ai, Br, Bi = list(Rr), list(Ri), list(reversed(Rr)), list(reversed(Ri))
+    for k in (1, 40, 75, 98):
+        Ar[k], Br[k], Ai[k], Bi[k] = v(top | 1, 0), v(top | 1, 0), v(top, 0), v(top, -pb, -1)     # Re on a midpoint
+        Ar[k + 1], Ai[k + 1], Br[k + 1], Bi[k + 1] = v(top | 5, 3), v(top | 5, 3), v(top | 9, -2), v(top | 9, -2)  # Re = +0
+        Ar[k - 1], Ai[k - 1], Br[k - 1], Bi[k - 1] = v(top | 5, 3), v(top | 5, 3), v(top | 9, -2), v(top | 11, -2)
+    a = _cplx_from(pack(Ar, pb), pack(Ai, pb), pb)
+    b = _cplx_from(pack(Br, pb), pack(Bi, pb), pb)
+    assert same(M.elementwise("mul", a.cuda(), b.cuda()), O.elementwise_complex(pb, "mul", a.pair(), b.pair())) == 0
+    # AXPY with a = (2^(pb-1)+1, 2^(pb-1)) 2^-pb: x picks the same three cases
+    ca = _cplx_from(pack([v(top | 1, 0)], pb), pack([v(top, 0)], pb), pb)
+    Xr, Xi, Yr, Yi = list(Rr), list(Ri), list(Ri), list(Rr)
+    for k in (1, 40, 75, 98):
+        Xr[k], Xi[k] = v(top | 1, 0), v(top, -pb, -1)           # Re = a_r x_r - a_i x_i on a midpoint
+        Xr[k + 1], Xi[k + 1] = v(top, 7), v(top | 1, 7)         # Re = +0
+        Xr[k - 1], Xi[k - 1] = v(top, 7), v(top | 3, 7)         # Re = -2^(pb-1) 2^(14-2pb)
+        for j in (k - 1, k, k + 1):
+            Yr[j], Yi[j] = zero, zero
+    x = _cplx_from(pack(Xr, pb), pack(Xi, pb), pb)
+    y = _cplx_from(pack(Yr, pb), pack(Yi, pb), pb)
+    assert same(M.axpy(ca.cuda(), x.cuda(), y.cuda()), O.axpy_complex(pb, ca.pair(), x.pair(), y.pair())) == 0
+
+
 @pytest.mark.parametrize("pb", PBS)
 def test_gemm_gemv_perop_vs_golden_and_oracle(gpu, pb):
     g = goldens.load(pb)
