@@ -315,20 +315,6 @@ struct rres
   u32 k[KW];
 };
 
-template <int LW>
-CUFP_INL bool mag_ge (i64 ea, const u32 (&A)[LW], i64 eb, const u32 (&B)[LW])
-{
-  /* lexicographic (exp, words from the top); a select chain, no branches */
-  int c = 0;                                 /* result of word compare: -1,0,1 */
-  CUFP_UNROLL
-  for (int i = 0; i < LW; i++)
-    {
-      int ci = (A[i] > B[i]) ? 1 : ((A[i] < B[i]) ? -1 : 0);
-      c = ci ? ci : c;                         /* later (higher) words win */
-    }
-  return (ea != eb) ? (ea > eb) : (c >= 0);
-}
-
 /* ERRB > 0: the operands are lower or upper bounds of the true terms, off by
  * less than 2^ERRB units of the last word of the larger one (together), and
  * r.amb reports whether the true sum could round differently: that is when
@@ -341,8 +327,9 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
 {
   const int L = LW + 2;
   rres<KW> r;
-  /* order by magnitude: zero is the smallest */
-  bool age = zb || (!za && mag_ge<LW> (ea, Ai, eb, Bi));
+  /* order by exponent (zero is the smallest); with equal exponents the
+   * difference may come out negative, and is negated below */
+  bool age = zb || (!za && ea >= eb);
   u32 A[LW], B[LW];
   CUFP_UNROLL
   for (int i = 0; i < LW; i++)
@@ -403,6 +390,23 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
   CUFP_UNROLL
   for (int i = 0; i < LW; i++) D[i + 1] = addc (A[i], T[i + 1] ^ xm, c);
   D[L - 1] = addc (0u, T[L - 1] ^ xm, c);
+  /* A - B < 0: only with equal exponents (otherwise A > B), so nothing was
+   * shifted out and the window holds the exact difference in two's
+   * complement (top word all ones): negate it, B's sign */
+  {
+    const bool neg = sub && (D[L - 1] >> 31) != 0;
+    if (any_lane (neg))
+      {
+        u32 cn = 1;
+        CUFP_UNROLL
+        for (int i = 0; i < L; i++)
+          {
+            const u32 v = addc (~D[i], 0u, cn);
+            D[i] = neg ? v : D[i];
+          }
+        sA = neg ? sB : sA;
+      }
+  }
 
   /* carry into the top word: shift right by one, dropped bit -> sticky */
   u32 cy = D[L - 1] & 1u;
@@ -414,17 +418,23 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
       u32 hi = D[i + 1];
       E[i] = cy ? fshr (hi, D[i], 1u) : D[i];
     }
-  /* leading zeros of E (L-1 words) */
-  u32 lz = 0;
-  bool seen = false;
-  CUFP_UNROLL
-  for (int i = L - 2; i >= 0; i--)
+  /* leading zeros of E (L-1 words); a full scan only when some lane has
+   * cancelled a whole word */
+  u32 lz = (u32) clz32 (E[L - 2]);
+  bool zres = false;
+  if (any_lane (E[L - 2] == 0))
     {
-      u32 z = (u32) clz32 (E[i]);
-      lz += seen ? 0u : z;
-      seen = seen || (E[i] != 0);
+      lz = 0;
+      bool seen = false;
+      CUFP_UNROLL
+      for (int i = L - 2; i >= 0; i--)
+        {
+          u32 z = (u32) clz32 (E[i]);
+          lz += seen ? 0u : z;
+          seen = seen || (E[i] != 0);
+        }
+      zres = !seen;
     }
-  bool zres = !seen;
   u32 wl = lz >> 5, bl = lz & 31u;
   if (wl != 0)
     {
