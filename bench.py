@@ -236,7 +236,8 @@ def extra_configs(args):
     """Every other configuration, on GPU-generated inputs: real AXPY over the
     precision sweep (n = 2^20), GEMV 4096^2 at 512 bits, the exact GEMM with
     exponent spread [-80, 80], the complex GEMM 1024^3 at 256 bits (per-op and
-    exact), and the 10,000-instance batched workload."""
+    exact), division and square root at 256 and 1024 bits (n = 2^18), and the
+    10,000-instance batched workload."""
     import mpc_cuda_amd as M
     out = {}
     for pb in (64, 128, 512, 1024, 2048):
@@ -272,6 +273,18 @@ def extra_configs(args):
     out["cgemm_pb256_1024^3"] = {"perop_ms": ms_p, "exact_ms": ms_e, "exact_speedup": ms_p / ms_e,
                                  "exact_MP_ops_per_s": 2 * d ** 3 / (ms_e * 1e-3)}
     del A, B
+    # division and square root (cu_fdiv / cu_sqrt: Newton at pb + 128 bits),
+    # n = 2^18 values, results per second
+    for pb in (256, 1024):
+        n = 1 << 18
+        a = M.device_random(pb, n, 601, -30, 30)
+        b = M.device_random(pb, n, 602, -30, 30)
+        o = M.empty(pb, n)
+        ms_d = _timed(lambda: M.elementwise("div", a, b, out=o), 5)
+        ms_s = _timed(lambda: M.sqrt(a, out=o), 5)
+        out[f"div_sqrt_pb{pb}_n2^18"] = {"div_ms": ms_d, "divisions_per_s": n / (ms_d * 1e-3),
+                                         "sqrt_ms": ms_s, "square_roots_per_s": n / (ms_s * 1e-3)}
+        del a, b, o
     out["batch_10000"] = batch_config(args)
     return out
 

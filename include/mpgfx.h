@@ -52,7 +52,7 @@ enum mpgfx_status {
   MPGFX_ENOMEM = 4         /* workspace allocation failed */
 };
 
-enum mpgfx_op { MPGFX_MUL = 0, MPGFX_ADD = 1, MPGFX_SUB = 2 };
+enum mpgfx_op { MPGFX_MUL = 0, MPGFX_ADD = 1, MPGFX_SUB = 2, MPGFX_DIV = 3 };
 enum mpgfx_gemm_mode { MPGFX_GEMM_PEROP = 0, MPGFX_GEMM_EXACT = 1 };
 
 /* Library version (major*10000 + minor*100 + patch). */
@@ -67,15 +67,23 @@ const char *mpgfx_last_error (void);
 
 /* o[i] = a[i] OP b[i] for real MP arrays of n values, OP in {mul, add, sub},
  * each RNDN.  Replaces the reference's per-element use of cu_fmul / cu_fadd /
- * cu_fsub: tools/test_freal.cu:41-48 (kern<PB>), cu_freal.cuh:203-213, 280-292. */
+ * cu_fsub: tools/test_freal.cu:41-48 (kern<PB>), cu_freal.cuh:203-213, 280-292.
+ * OP = div: a[i] / b[i] as cu_fdiv (cu_fmath.cuh:68-75): faithful, not
+ * correctly rounded (Newton refinement at pb + 128 bits, bit-identical to the
+ * reference); a zero divisor gives +0. */
 int mpgfx_elementwise_real (int pb, int op, int64_t n,
                             const uint64_t *a_limbs, const int64_t *a_se,
                             const uint64_t *b_limbs, const int64_t *b_se,
                             uint64_t *o_limbs, int64_t *o_se, void *stream);
 
+/* o[i] = sqrt(x[i]) for a real MP array, as cu_sqrt (cu_fmath.cuh:78-100):
+ * faithful, bit-identical to the reference; +0 for zero and negative x. */
+int mpgfx_sqrt_real (int pb, int64_t n, const uint64_t *x_limbs, const int64_t *x_se,
+                     uint64_t *o_limbs, int64_t *o_se, void *stream);
+
 /* Complex counterpart, MPC_RNDNN per component: cu_cmul / cu_cadd / cu_csub
- * applied elementwise; replaces tools/test_fcomplex.cu:147-153 (kern<PB>),
- * cu_fcomplex.cuh:115-124. */
+ * applied elementwise (OP = div is not available); replaces
+ * tools/test_fcomplex.cu:147-153 (kern<PB>), cu_fcomplex.cuh:115-124. */
 int mpgfx_elementwise_complex (int pb, int op, int64_t n,
                                const uint64_t *a_limbs, const int64_t *a_se,
                                const uint64_t *b_limbs, const int64_t *b_se,

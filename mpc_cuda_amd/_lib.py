@@ -9,7 +9,7 @@ LIB_PATH = os.path.join(HERE, "libmpgfx.so")
 
 OK, EINVAL, EUNSUPPORTED, EHIP, ENOMEM = 0, 1, 2, 3, 4
 STATUS = {OK: "OK", EINVAL: "EINVAL", EUNSUPPORTED: "EUNSUPPORTED", EHIP: "EHIP", ENOMEM: "ENOMEM"}
-MUL, ADD, SUB = 0, 1, 2
+MUL, ADD, SUB, DIV = 0, 1, 2, 3
 GEMM_PEROP, GEMM_EXACT = 0, 1
 
 # every symbol include/mpgfx.h declares, with its ctypes signature
@@ -22,6 +22,7 @@ SIGNATURES = {
     "mpgfx_last_error": (ctypes.c_char_p, []),
     "mpgfx_elementwise_real": (_i, [_i, _i, _l] + [_p] * 6 + [_p]),
     "mpgfx_elementwise_complex": (_i, [_i, _i, _l] + [_p] * 6 + [_p]),
+    "mpgfx_sqrt_real": (_i, [_i, _l] + [_p] * 4 + [_p]),
     "mpgfx_axpy_real": (_i, [_i, _l] + [_p] * 8 + [_p]),
     "mpgfx_axpy_complex": (_i, [_i, _l] + [_p] * 8 + [_p]),
     "mpgfx_gemv_real": (_i, [_i, _l, _l] + [_p] * 6 + [_p]),

@@ -46,11 +46,21 @@ extern "C" int mpgfx_elementwise_real (int pb, int op, int64_t n, const uint64_t
                                        const uint64_t *bl, const int64_t *bse, uint64_t *ol, int64_t *ose,
                                        void *stream)
 {
-  if (n < 0 || op < MPGFX_MUL || op > MPGFX_SUB) return MPGFX_EINVAL;
+  if (n < 0 || op < MPGFX_MUL || op > MPGFX_DIV) return MPGFX_EINVAL;
   GET_OPS (o);
   if (n == 0) return MPGFX_OK;
   NEED (al && ase && bl && bse && ol && ose);
   return o->ew_real (op, n, al, ase, bl, bse, ol, ose, (hipStream_t) stream);
+}
+
+extern "C" int mpgfx_sqrt_real (int pb, int64_t n, const uint64_t *xl, const int64_t *xse, uint64_t *ol,
+                                int64_t *ose, void *stream)
+{
+  if (n < 0) return MPGFX_EINVAL;
+  GET_OPS (o);
+  if (n == 0) return MPGFX_OK;
+  NEED (xl && xse && ol && ose);
+  return o->sqrt_real (n, xl, xse, ol, ose, (hipStream_t) stream);
 }
 
 extern "C" int mpgfx_elementwise_complex (int pb, int op, int64_t n, const uint64_t *al, const int64_t *ase,

@@ -13,6 +13,7 @@ struct pb_ops
   int pb;
   int (*ew_real) (int op, i64 n, const u64 *, const i64 *, const u64 *, const i64 *, u64 *, i64 *, hipStream_t);
   int (*ew_cplx) (int op, i64 n, const u64 *, const i64 *, const u64 *, const i64 *, u64 *, i64 *, hipStream_t);
+  int (*sqrt_real) (i64 n, const u64 *, const i64 *, u64 *, i64 *, hipStream_t);
   int (*axpy_real) (i64 n, const u64 *, const i64 *, const u64 *, const i64 *, const u64 *, const i64 *, u64 *,
                     i64 *, hipStream_t);
   int (*axpy_cplx) (i64 n, const u64 *, const i64 *, const u64 *, const i64 *, const u64 *, const i64 *, u64 *,

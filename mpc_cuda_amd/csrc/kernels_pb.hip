@@ -42,9 +42,19 @@ __global__ __launch_bounds__ (TPB) void k_ew_real (i64 n, const u64 *__restrict_
       cu_freal<PB> a = ld<PB> (al, ase, n, i), b = ld<PB> (bl, bse, n, i), r;
       if (OP == MPGFX_MUL) r = cu_fp::cu_fmul<PB> (a, b);
       else if (OP == MPGFX_ADD) r = cu_fp::cu_fadd<PB> (a, b);
-      else r = cu_fp::cu_fsub<PB> (a, b);
+      else if (OP == MPGFX_SUB) r = cu_fp::cu_fsub<PB> (a, b);
+      else r = cu_fp::cu_fdiv<PB> (a, b);
       st<PB> (r, ol, ose, n, i);
     }
+}
+
+/* o = sqrt(x) per element (cu_sqrt: Newton at PB + 128 bits) */
+__global__ __launch_bounds__ (TPB) void k_sqrt_real (i64 n, const u64 *__restrict__ xl, const i64 *__restrict__ xse,
+                                                     u64 *__restrict__ ol, i64 *__restrict__ ose)
+{
+  i64 stride = (i64) gridDim.x * blockDim.x;
+  for (i64 i = (i64) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    st<PB> (cu_fp::cu_sqrt<PB> (ld<PB> (xl, xse, n, i)), ol, ose, n, i);
 }
 
 template <int OP>
@@ -331,8 +341,14 @@ int ew_real (int op, i64 n, const u64 *al, const i64 *ase, const u64 *bl, const 
   unsigned g = grid_for (n, TPB, 32);
   if (op == MPGFX_MUL) hipLaunchKernelGGL ((k_ew_real<MPGFX_MUL>), dim3 (g), dim3 (TPB), 0, s, n, al, ase, bl, bse, ol, ose);
   else if (op == MPGFX_ADD) hipLaunchKernelGGL ((k_ew_real<MPGFX_ADD>), dim3 (g), dim3 (TPB), 0, s, n, al, ase, bl, bse, ol, ose);
-  else hipLaunchKernelGGL ((k_ew_real<MPGFX_SUB>), dim3 (g), dim3 (TPB), 0, s, n, al, ase, bl, bse, ol, ose);
+  else if (op == MPGFX_SUB) hipLaunchKernelGGL ((k_ew_real<MPGFX_SUB>), dim3 (g), dim3 (TPB), 0, s, n, al, ase, bl, bse, ol, ose);
+  else hipLaunchKernelGGL ((k_ew_real<MPGFX_DIV>), dim3 (g), dim3 (TPB), 0, s, n, al, ase, bl, bse, ol, ose);
   return launch_status ("elementwise_real");
+}
+int sqrt_real (i64 n, const u64 *xl, const i64 *xse, u64 *ol, i64 *ose, hipStream_t s)
+{
+  hipLaunchKernelGGL (k_sqrt_real, dim3 (grid_for (n, TPB, 32)), dim3 (TPB), 0, s, n, xl, xse, ol, ose);
+  return launch_status ("sqrt_real");
 }
 int ew_cplx (int op, i64 n, const u64 *al, const i64 *ase, const u64 *bl, const i64 *bse, u64 *ol, i64 *ose,
              hipStream_t s)
@@ -438,7 +454,7 @@ int unpack (int cplx, i64 n, const u64 *l, const i64 *se, void *aos, hipStream_t
 #include "exact_pb.h"
 #include "batch_pb.h"
 
-const pb_ops ops = { PB, ew_real, ew_cplx, axpy_real, axpy_cplx, gemm_perop, gemm_cplx, gemv, pack, unpack,
+const pb_ops ops = { PB, ew_real, ew_cplx, sqrt_real, axpy_real, axpy_cplx, gemm_perop, gemm_cplx, gemv, pack, unpack,
                      exact_digits_launch, exact_round_launch, exact_fallback_launch, batch_launch };
 
 } // namespace

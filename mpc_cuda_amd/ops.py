@@ -29,20 +29,34 @@ def _same(pb, *xs):
             raise ValueError("precision mismatch")
 
 
-_OPS = {"mul": _lib.MUL, "add": _lib.ADD, "sub": _lib.SUB}
+_OPS = {"mul": _lib.MUL, "add": _lib.ADD, "sub": _lib.SUB, "div": _lib.DIV}
 
 
 def elementwise(op, a, b, out=None):
-    """out = a OP b elementwise (real: cu_fmul/cu_fadd/cu_fsub; complex:
-    cu_cmul/cu_cadd/cu_csub)."""
+    """out = a OP b elementwise (real: cu_fmul/cu_fadd/cu_fsub, and cu_fdiv for
+    "div" -- faithful, Newton at pb + 128 bits; complex: cu_cmul/cu_cadd/cu_csub)."""
     _dev(a), _dev(b)
     _same(a.pb, b)
     if a.cplx != b.cplx or a.n != b.n:
         raise ValueError("shape mismatch")
+    if a.cplx and op == "div":
+        raise ValueError("complex division is not provided")
     out = out if out is not None else empty(a.pb, a.n, a.cplx, device=a.se.device)
     L = _lib.lib()
     f = L.mpgfx_elementwise_complex if a.cplx else L.mpgfx_elementwise_real
     _lib.check(f.__name__, f(a.pb, _OPS[op], a.n, *a.ptrs(), *b.ptrs(), *out.ptrs(), _stream()))
+    return out
+
+
+def sqrt(x, out=None):
+    """out = cu_sqrt(x) elementwise for a real array (faithful; +0 for zero and
+    negative values)."""
+    _dev(x)
+    if x.cplx:
+        raise ValueError("complex square root is not provided")
+    out = out if out is not None else empty(x.pb, x.n, False, device=x.se.device)
+    L = _lib.lib()
+    _lib.check("mpgfx_sqrt_real", L.mpgfx_sqrt_real(x.pb, x.n, *x.ptrs(), *out.ptrs(), _stream()))
     return out
 
 

@@ -382,6 +382,105 @@ double or_to_double (int pb, const or_real *x)
   return (double) x->sign * ldexp ((double) F, (int) (e - 53));
 }
 
+/* ---------------- division and square root (cu_fmath.cuh:25-100) ----------------
+ * Faithful, not correctly rounded: Newton refinement at W = pb + 128 bits from
+ * a double seed, every operation RNDN at W, the result rounded to pb. */
+#define OR_FGUARD 128                   /* cu_fmath.cuh:21 */
+
+/* exact small integer (cu_freal.cuh:326-334) */
+static void or_from_si (int pb, long v, or_real *r)
+{
+  if (v == 0) { or_set_zero (pb, r); return; }
+  int sgn = v < 0 ? -1 : 1;
+  uint64_t a = v < 0 ? (uint64_t) (-(v + 1)) + 1u : (uint64_t) v;
+  int nb = 64 - __builtin_clzll (a);
+  or_finalize (pb, &a, 1, nb - 1, nb - 1, 0, 0, sgn, r);
+}
+
+/* re-round from precision `from` to `to` (cu_freal.cuh:317-323) */
+static void or_convert (int to, int from, const or_real *x, or_real *r)
+{
+  if (is_zero (x)) { or_set_zero (to, r); return; }
+  int N = or_nlimbs (from);
+  or_finalize (to, x->m, N, 64L * N - 1, x->exp - 1, 0, 0, x->sign, r);
+}
+
+/* steps from a 52-bit seed to W bits, the seed counted (cu_fmath.cuh:25-26) */
+static int or_newton_iters (int W)
+{
+  int k = 1;
+  for (long p = 52; p < W; p += p) k++;
+  return k;
+}
+
+/* 1/b at precision W (cu_fmath.cuh:53-66) */
+static void or_recip_w (int W, const or_real *b, or_real *z)
+{
+  or_real m = *b, two, t, u;
+  int sgn = b->sign;
+  int64_t e = b->exp;
+  m.sign = 1; m.exp = 0;
+  or_from_double (W, 1.0 / or_to_double (W, &m), z);
+  or_from_si (W, 2, &two);
+  int it = or_newton_iters (W) + 1;
+  for (int i = 0; i < it; i++)
+    {
+      or_mul (W, &m, z, &t);            /* m z       */
+      or_sub (W, &two, &t, &u);         /* 2 - m z   */
+      or_mul (W, z, &u, &t);            /* z (2-m z) */
+      *z = t;
+    }
+  z->sign = sgn;
+  z->exp -= e;
+}
+
+/* a / b (cu_fmath.cuh:68-75); a zero divisor gives +0 (the reference leaves it undefined) */
+void or_fdiv (int pb, const or_real *a, const or_real *b, or_real *r)
+{
+  if (is_zero (a) || is_zero (b)) { or_set_zero (pb, r); return; }
+  int W = pb + OR_FGUARD;
+  or_real aw, bw, z, q;
+  or_convert (W, pb, a, &aw);
+  or_convert (W, pb, b, &bw);
+  or_recip_w (W, &bw, &z);
+  or_mul (W, &aw, &z, &q);
+  or_convert (pb, W, &q, r);
+}
+
+/* sqrt(x), x > 0, at precision W (cu_fmath.cuh:78-92) */
+static void or_sqrt_w (int W, const or_real *x, or_real *r)
+{
+  int64_t e = x->exp, t;
+  or_real m = *x, y, three, y2, p, d, q;
+  m.sign = 1;
+  if (e & 1) { m.exp = 1; t = (e - 1) / 2; } else { m.exp = 0; t = e / 2; }
+  or_from_double (W, 1.0 / sqrt (or_to_double (W, &m)), &y);
+  or_from_si (W, 3, &three);
+  int it = or_newton_iters (W) + 1;
+  for (int i = 0; i < it; i++)
+    {
+      or_mul (W, &y, &y, &y2);          /* y^2               */
+      or_mul (W, &m, &y2, &p);          /* m y^2             */
+      or_sub (W, &three, &p, &d);       /* 3 - m y^2         */
+      or_mul (W, &y, &d, &q);           /* y (3 - m y^2)     */
+      if (!is_zero (&q)) q.exp -= 1;    /* / 2 (cu_scale2)   */
+      y = q;
+    }
+  or_mul (W, &m, &y, r);
+  r->exp += t;
+}
+
+/* sqrt(x); +0 for zero and negative x (cu_fmath.cuh:93-100) */
+void or_fsqrt (int pb, const or_real *x, or_real *r)
+{
+  if (is_zero (x) || x->sign < 0) { or_set_zero (pb, r); return; }
+  int W = pb + OR_FGUARD;
+  or_real xw, s;
+  or_convert (W, pb, x, &xw);
+  or_sqrt_w (W, &xw, &s);
+  or_convert (pb, W, &s, r);
+}
+
 /* ---------------- reference generator, tools/test_freal.cu:15-27 ---------------- */
 uint64_t or_xs (uint64_t *s)
 {
@@ -430,7 +529,21 @@ void or_elementwise_real (int pb, int op, int64_t n,
       or_load (pb, al, ase, n, i, &a); or_load (pb, bl, bse, n, i, &b);
       if (op == 0) or_mul (pb, &a, &b, &r);
       else if (op == 1) or_add (pb, &a, &b, &r);
-      else or_sub (pb, &a, &b, &r);
+      else if (op == 2) or_sub (pb, &a, &b, &r);
+      else or_fdiv (pb, &a, &b, &r);
+      or_store (pb, &r, ol, ose, n, i);
+    }
+}
+
+void or_sqrt_real (int pb, int64_t n, const uint64_t *xl, const int64_t *xse,
+                   uint64_t *ol, int64_t *ose)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++)
+    {
+      or_real x, r;
+      or_load (pb, xl, xse, n, i, &x);
+      or_fsqrt (pb, &x, &r);
       or_store (pb, &r, ol, ose, n, i);
     }
 }

@@ -63,10 +63,16 @@ void or_load (int pb, const uint64_t *limbs, const int64_t *se, int64_t n,
               int64_t i, or_real *r);
 void or_store (int pb, const or_real *r, uint64_t *limbs, int64_t *se,
                int64_t n, int64_t i);
-/* op: 0 = mul, 1 = add, 2 = sub (tools/test_freal.cu:41-48) */
+/* division and square root at working precision pb + 128 (cu_fmath.cuh:53-100) */
+void or_fdiv (int pb, const or_real *a, const or_real *b, or_real *r);
+void or_fsqrt (int pb, const or_real *x, or_real *r);
+/* op: 0 = mul, 1 = add, 2 = sub (tools/test_freal.cu:41-48), 3 = div (cu_fdiv) */
 void or_elementwise_real (int pb, int op, int64_t n,
     const uint64_t *al, const int64_t *ase, const uint64_t *bl, const int64_t *bse,
     uint64_t *ol, int64_t *ose);
+/* out = sqrt(x) per element (cu_sqrt) */
+void or_sqrt_real (int pb, int64_t n, const uint64_t *xl, const int64_t *xse,
+                   uint64_t *ol, int64_t *ose);
 /* op on complex arrays (tools/test_fcomplex.cu:147-153) */
 void or_elementwise_complex (int pb, int op, int64_t n,
     const uint64_t *al, const int64_t *ase, const uint64_t *bl, const int64_t *bse,

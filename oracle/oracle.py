@@ -56,6 +56,7 @@ def lib():
         L.or_axpy_complex.argtypes = [ctypes.c_int, ctypes.c_int64] + arr2 * 4
         for f in (L.or_gemm_perop, L.or_gemm_exact, L.or_cgemm_perop):
             f.argtypes = [ctypes.c_int] + [ctypes.c_int64] * 3 + arr2 * 3 + [ctypes.c_int64] * 2
+        L.or_sqrt_real.argtypes = [ctypes.c_int, ctypes.c_int64] + arr2 * 2
         L.or_fill_ref.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int, _u64p, _i64p]
         L.or_num_threads.restype = ctypes.c_int
         _LIB = L
@@ -78,6 +79,7 @@ def ref():
         L.ref_elementwise_complex.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64] + arr2 * 3
         L.ref_axpy_real.argtypes = [ctypes.c_int, ctypes.c_int64] + arr2 * 4
         L.ref_axpy_complex.argtypes = [ctypes.c_int, ctypes.c_int64] + arr2 * 4
+        L.ref_sqrt_real.argtypes = [ctypes.c_int, ctypes.c_int64] + arr2 * 2
         L.ref_gemm_perop.argtypes = [ctypes.c_int] + [ctypes.c_int64] * 3 + arr2 * 3 + [ctypes.c_int64] * 2
         L.ref_supported.argtypes = [ctypes.c_int]
         L.ref_num_threads.restype = ctypes.c_int
@@ -106,7 +108,7 @@ def fill_ref(pb, n, seed, spread):
     return limbs, se
 
 
-OPS = {"mul": 0, "add": 1, "sub": 2}
+OPS = {"mul": 0, "add": 1, "sub": 2, "div": 3}
 
 
 def elementwise_real(pb, op, a, b, impl="oracle"):
@@ -119,6 +121,19 @@ def elementwise_real(pb, op, a, b, impl="oracle"):
         lib().or_elementwise_real(*args)
     else:
         assert ref().ref_elementwise_real(*args) == 0
+    return ol, ose
+
+
+def sqrt_real(pb, x, impl="oracle"):
+    """cu_sqrt per element (faithful; +0 for zero and negative values)."""
+    xl, xse = x
+    n = xse.shape[0]
+    ol, ose = empty_real(pb, n)
+    args = (pb, n, _p(xl, _u64p), _p(xse, _i64p), _p(ol, _u64p), _p(ose, _i64p))
+    if impl == "oracle":
+        lib().or_sqrt_real(*args)
+    else:
+        assert ref().ref_sqrt_real(*args) == 0
     return ol, ose
 
 

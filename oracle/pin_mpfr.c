@@ -10,7 +10,8 @@
  *   FILE_IN holds the operands as consecutive SoA arrays (see mpfixed.h);
  *   FILE_OUT receives the result array.  Zero results are written as
  *   se = ZERO*2 + (MPFR sign bit), limbs 0.
- *   OP: mul add sub            (n = M)        a[n], b[n]        -> o[n]
+ *   OP: mul add sub div        (n = M)        a[n], b[n]        -> o[n]
+ *       sqrt                   (n = M)        x[n]              -> o[n]  (x > 0)
  *       cmul cadd csub         (n = M)        complex a, b      -> o
  *       axpy                   (n = M)        a[1], x[n], y[n]  -> o[n]
  *       caxpy                  (n = M)        complex a[1], x, y -> o
@@ -126,7 +127,7 @@ int main (int argc, char **argv)
   int64_t Nn = argc > 6 ? atoll (argv[6]) : 0, K = argc > 7 ? atoll (argv[7]) : 0;
   (void) to_mpc;
 
-  if (!strcmp (op, "mul") || !strcmp (op, "add") || !strcmp (op, "sub"))
+  if (!strcmp (op, "mul") || !strcmp (op, "add") || !strcmp (op, "sub") || !strcmp (op, "div"))
     {
       arr a = read_arr (fi, M), b = read_arr (fi, M), o = new_arr (M);
 #pragma omp parallel
@@ -138,10 +139,28 @@ int main (int argc, char **argv)
             to_mpfr (x, a.l, a.se, M, i); to_mpfr (y, b.l, b.se, M, i);
             if (op[0] == 'm') mpfr_mul (r, x, y, MPFR_RNDN);
             else if (op[0] == 'a') mpfr_add (r, x, y, MPFR_RNDN);
+            else if (op[0] == 'd') mpfr_div (r, x, y, MPFR_RNDN);
             else mpfr_sub (r, x, y, MPFR_RNDN);
             from_mpfr (r, o.l, o.se, M, i);
           }
         mpfr_clears (x, y, r, (mpfr_ptr) 0);
+      }
+      write_arr (fo, o);
+    }
+  else if (!strcmp (op, "sqrt"))
+    {
+      arr a = read_arr (fi, M), o = new_arr (M);
+#pragma omp parallel
+      {
+        mpfr_t x, r; mpfr_inits2 (pb, x, r, (mpfr_ptr) 0);
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < M; i++)
+          {
+            to_mpfr (x, a.l, a.se, M, i);
+            mpfr_sqrt (r, x, MPFR_RNDN);
+            from_mpfr (r, o.l, o.se, M, i);
+          }
+        mpfr_clears (x, r, (mpfr_ptr) 0);
       }
       write_arr (fo, o);
     }

@@ -16,6 +16,7 @@
  */
 #include "mpc_cuda/cu_freal.cuh"
 #include "mpc_cuda/cu_fcomplex.cuh"
+#include "mpc_cuda/cu_fmath.cuh"
 #include <cstdint>
 #include <cstddef>
 #include <cstdlib>
@@ -61,9 +62,15 @@ template <int PB> void ew_real (int op, int64_t n, const uint64_t *al, const int
       cu_freal<PB> a = ld<PB> (al, ase, n, i), b = ld<PB> (bl, bse, n, i), r;
       if (op == 0) r = cu_fp::cu_fmul<PB> (a, b);
       else if (op == 1) r = cu_fp::cu_fadd<PB> (a, b);
-      else r = cu_fp::cu_fsub<PB> (a, b);
+      else if (op == 2) r = cu_fp::cu_fsub<PB> (a, b);
+      else r = cu_fp::cu_fdiv<PB> (a, b);
       st<PB> (r, ol, ose, n, i);
     }
+}
+template <int PB> void sqrt_real (int64_t n, const uint64_t *xl, const int64_t *xse, uint64_t *ol, int64_t *ose)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) st<PB> (cu_fp::cu_sqrt<PB> (ld<PB> (xl, xse, n, i)), ol, ose, n, i);
 }
 template <int PB> void ew_cplx (int op, int64_t n, const uint64_t *al, const int64_t *ase,
                                 const uint64_t *bl, const int64_t *bse, uint64_t *ol, int64_t *ose)
@@ -149,6 +156,13 @@ int ref_elementwise_real (int pb, int op, int64_t n, const uint64_t *al, const i
                           const uint64_t *bl, const int64_t *bse, uint64_t *ol, int64_t *ose)
 {
 #define C(P) if (pb == P) { ew_real<P> (op, n, al, ase, bl, bse, ol, ose); return 0; }
+  PB_LIST (C)
+#undef C
+  return -1;
+}
+int ref_sqrt_real (int pb, int64_t n, const uint64_t *xl, const int64_t *xse, uint64_t *ol, int64_t *ose)
+{
+#define C(P) if (pb == P) { sqrt_real<P> (n, xl, xse, ol, ose); return 0; }
   PB_LIST (C)
 #undef C
   return -1;

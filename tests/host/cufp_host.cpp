@@ -27,7 +27,7 @@ template <int PB> void ew_real (int op, int64_t n, const uint64_t *al, const int
   for (int64_t i = 0; i < n; i++)
     {
       cu_freal<PB> a = ld<PB> (al, ase, n, i), b = ld<PB> (bl, bse, n, i);
-      st<PB> (op == 0 ? a * b : (op == 1 ? a + b : a - b), ol, ose, n, i);
+      st<PB> (op == 0 ? a * b : (op == 1 ? a + b : (op == 2 ? a - b : cu_fdiv<PB> (a, b))), ol, ose, n, i);
     }
 }
 template <int PB> void ew_cplx (int op, int64_t n, const uint64_t *al, const int64_t *ase,
@@ -43,6 +43,11 @@ template <int PB> void ew_cplx (int op, int64_t n, const uint64_t *al, const int
       st<PB> (r.re, ol, ose, n, i);
       st<PB> (r.im, ol + o, ose + n, n, i);
     }
+}
+template <int PB> void sqrt_real (int64_t n, const uint64_t *xl, const int64_t *xse, uint64_t *ol, int64_t *ose)
+{
+#pragma omp parallel for
+  for (int64_t i = 0; i < n; i++) st<PB> (cu_sqrt<PB> (ld<PB> (xl, xse, n, i)), ol, ose, n, i);
 }
 template <int PB> double conv (double d, int *exp_out)
 {
@@ -66,6 +71,13 @@ extern "C" int host_elementwise_complex (int pb, int op, int64_t n, const uint64
                                          const uint64_t *bl, const int64_t *bse, uint64_t *ol, int64_t *ose)
 {
 #define C(P) if (pb == P) { ew_cplx<P> (op, n, al, ase, bl, bse, ol, ose); return 0; }
+  PB_LIST (C)
+#undef C
+  return -1;
+}
+extern "C" int host_sqrt_real (int pb, int64_t n, const uint64_t *xl, const int64_t *xse, uint64_t *ol, int64_t *ose)
+{
+#define C(P) if (pb == P) { sqrt_real<P> (n, xl, xse, ol, ose); return 0; }
   PB_LIST (C)
 #undef C
   return -1;
