@@ -40,6 +40,30 @@ __global__ __launch_bounds__ (256) void k_peak (int iters, uint64_t seed, uint64
   out[(size_t) blockIdx.x * blockDim.x + threadIdx.x] = x;
 }
 
+/* the same 8 chains with LDSB bytes of LDS per 256-lane workgroup, which caps
+ * the workgroups per CU and so the wavefronts per SIMD (96 KB: one, 64 KB:
+ * two, 4 KB: eight); the LDS is allocated, never used in the loop */
+template <int LDSB>
+__global__ __launch_bounds__ (256) void k_mad_issue (int iters, uint64_t seed, uint64_t *out)
+{
+  __shared__ u32 hog[LDSB / 4];
+  if (seed == 0) hog[threadIdx.x] = 1;              /* never taken: keeps the allocation */
+  u32 a = (u32) (seed + threadIdx.x), b = (u32) (seed >> 32) ^ blockIdx.x;
+  u64 acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) acc[j] = seed * (j + 1) + threadIdx.x;
+  for (int it = 0; it < iters; it++)
+    {
+#pragma unroll
+      for (int j = 0; j < 8; j++) acc[j] = (u64) (a + j) * (b ^ (u32) it) + acc[j];
+    }
+  u64 x = 0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) x ^= acc[j];
+  if (seed == 0) x += hog[threadIdx.x & 63];
+  out[(size_t) blockIdx.x * blockDim.x + threadIdx.x] = x;
+}
+
 constexpr int W = 32;        /* 1024-bit operands */
 
 __global__ __launch_bounds__ (256) void k_register (int pairs, int reps, const u32 *A, const u32 *B, uint64_t *out)
@@ -134,6 +158,27 @@ int main (int argc, char **argv)
     printf ("{\"bench\": \"mad_u64_u32_peak\", \"cus\": %d, \"clock_ghz\": %.3f, \"ms\": %.4f, "
             "\"word_products_per_s\": %.4e, \"per_cu_per_clk\": %.3f}\n",
             cus, clk_ghz, ms, mads / (ms * 1e-3), mads / (ms * 1e-3) / cus / (clk_ghz * 1e9));
+    HIPCHK (hipFree (out));
+  }
+  /* ---- v_mad_u64_u32 issue rate at 1, 2 and 8 wavefronts per SIMD ---- */
+  {
+    const int iters = 1 << 14;
+    uint64_t *out;
+    HIPCHK (hipMalloc (&out, (size_t) cus * 8 * 256 * 8));
+    auto one = [&] (int waves, int blocks, auto kern) {
+      hipLaunchKernelGGL (kern, dim3 (blocks), dim3 (256), 0, 0, 16, 12345ull, out);
+      HIPCHK (hipEventRecord (e0));
+      hipLaunchKernelGGL (kern, dim3 (blocks), dim3 (256), 0, 0, iters, 12345ull, out);
+      HIPCHK (hipEventRecord (e1));
+      HIPCHK (hipEventSynchronize (e1));
+      HIPCHK (hipEventElapsedTime (&ms, e0, e1));
+      double per_simd = (double) blocks * 4 * iters * 8 / (cus * 4.0);     /* wave-instructions per SIMD */
+      printf ("{\"bench\": \"mad_u64_u32_issue\", \"waves_per_simd\": %d, \"ms\": %.4f, "
+              "\"cycles_per_mad_per_simd\": %.2f}\n", waves, ms, (ms * 1e-3) * clk_ghz * 1e9 / per_simd);
+    };
+    one (1, cus, k_mad_issue<98304>);
+    one (2, cus * 2, k_mad_issue<65536>);
+    one (8, cus * 8, k_mad_issue<4096>);
     HIPCHK (hipFree (out));
   }
   /* ---- INT8 MFMA peak ---- */
