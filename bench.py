@@ -236,7 +236,8 @@ def extra_configs(args):
     """Every other configuration, on GPU-generated inputs: real AXPY over the
     precision sweep (n = 2^20), GEMV 4096^2 at 512 bits, the exact GEMM with
     exponent spread [-80, 80], the complex GEMM 1024^3 at 256 bits (per-op and
-    exact), division and square root at 256 and 1024 bits (n = 2^18), and the
+    exact), division and square root at 256 and 1024 bits (n = 2^18), the
+    run-time precision AXPY at 1024 and 1000 bits (n = 16384), and the
     10,000-instance batched workload."""
     import mpc_cuda_amd as M
     out = {}
@@ -285,6 +286,23 @@ def extra_configs(args):
         out[f"div_sqrt_pb{pb}_n2^18"] = {"div_ms": ms_d, "divisions_per_s": n / (ms_d * 1e-3),
                                          "sqrt_ms": ms_s, "square_roots_per_s": n / (ms_s * 1e-3)}
         del a, b, o
+    # run-time precision AXPY (the reference's MPFR / MPC device demos,
+    # N = 16384): at 1024 bits and at 1000 bits (not a compiled precision);
+    # host-generated inputs
+    for p in (1024, 1000):
+        n = 16384
+        a = M.random_real(p, 1, 701, -8, 8).cuda()
+        x = M.random_real(p, n, 702, -8, 8).cuda()
+        y = M.random_real(p, n, 703, -8, 8).cuda()
+        ms_r = _timed(lambda: M.rt_axpy(a, x, y), 10)
+        ca = M.random_complex(p, 1, 704, -8, 8).cuda()
+        cx = M.random_complex(p, n, 705, -8, 8).cuda()
+        cy = M.random_complex(p, n, 706, -8, 8).cuda()
+        ms_c = _timed(lambda: M.rt_axpy(ca, cx, cy), 10)
+        out[f"rt_axpy_p{p}_n16384"] = {"real_ms": ms_r, "complex_ms": ms_c,
+                                       "real_MP_ops_per_s": 2 * n / (ms_r * 1e-3),
+                                       "complex_MP_ops_per_s": 2 * n / (ms_c * 1e-3)}
+        del a, x, y, ca, cx, cy
     out["batch_10000"] = batch_config(args)
     return out
 

@@ -296,6 +296,41 @@ CUFP_INL u32 round_top (u32 (&K)[KW], const u32 (&D)[LEN], u32 s, u32 xst)
 }
 
 /* ---------------------------------------------------------------------------
+ * The same rounding to the top p bits of D (0 < p <= 32 KW, a run-time
+ * precision): K gets the top KW words of the result with the bits below the
+ * kept p cleared.  Word indices depend on p, so every word is visited with
+ * selects (no register indexing at run time).
+ * ------------------------------------------------------------------------- */
+template <int KW, int LEN>
+CUFP_INL u32 round_top_p (u32 (&K)[KW], const u32 (&D)[LEN], int p, u32 xst)
+{
+  static_assert (LEN > KW, "needs a guard word below the kept ones");
+  const int pos = 32 * LEN - p;                   /* bit index of the kept LSB */
+  const int pw = pos >> 5, pbit = pos & 31;
+  const int rw = (pos - 1) >> 5, rb = (pos - 1) & 31;
+  u32 rword = 0, lword = 0, sticky = xst;
+  CUFP_UNROLL
+  for (int t = 0; t < LEN; t++)
+    {
+      rword = (t == rw) ? D[t] : rword;
+      lword = (t == pw) ? D[t] : lword;
+      const u32 below = (t < rw) ? 0xffffffffu : ((t == rw) ? ((1u << rb) - 1u) : 0u);
+      sticky |= D[t] & below;
+    }
+  const u32 inc = ((rword >> rb) & 1u) & ((sticky != 0) | ((lword >> pbit) & 1u));
+  u32 c = 0;
+  CUFP_UNROLL
+  for (int j = 0; j < KW; j++)
+    {
+      const int t = LEN - KW + j;
+      const u32 keep = (t > pw) ? 0xffffffffu : ((t == pw) ? (0xffffffffu << pbit) : 0u);
+      K[j] = addc (D[t] & keep, (t == pw) ? (inc << pbit) : 0u, c);
+    }
+  K[KW - 1] |= c << 31;
+  return c;
+}
+
+/* ---------------------------------------------------------------------------
  * RNDN(A +/- B) for two exact operands of LW words each (top bit set unless
  * zero), value = sign * M * 2^(e - 32*LW), rounded to KW words.  The larger
  * magnitude is placed in a window of LW+2 words (one guard word below, one
@@ -320,10 +355,11 @@ struct rres
  * r.amb reports whether the true sum could round differently: that is when
  * it lies within that error of a rounding midpoint, or the computed sum is 0.
  * With 4 guard words below the kept ones (LW = KW + 3) the test reads the
- * top 64 guard bits. */
+ * top 64 guard bits.  p > 0 rounds to p bits (a run-time precision, the
+ * operands' low bits zero) rather than to KW words. */
 template <int KW, int LW, int ERRB = 0>
 CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
-                             int sb, i64 eb, bool zb, const u32 (&Bi)[LW])
+                             int sb, i64 eb, bool zb, const u32 (&Bi)[LW], int p = 0)
 {
   const int L = LW + 2;
   rres<KW> r;
@@ -478,7 +514,8 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
         }
       r.amb = amb || zres;
     }
-  u32 co = round_top<KW, L - 1> (r.k, E, 0u, st);
+  /* p > 0: round to p bits (run-time precision) instead of KW words */
+  u32 co = p ? round_top_p<KW, L - 1> (r.k, E, p, st) : round_top<KW, L - 1> (r.k, E, 0u, st);
   r.sign = zres ? 1 : sA;
   r.exp = eA + (i64) cy - (i64) lz + (i64) co;
   r.zero = zres;

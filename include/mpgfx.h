@@ -188,6 +188,32 @@ int mpgfx_import_mpfr (int pb, int64_t n, const void *x, size_t stride,
 int mpgfx_export_mpfr (int pb, int64_t n, const uint64_t *limbs, const int64_t *se,
                        void *x, size_t stride, void *stream);
 
+/* Run-time precision (any prec in 1 .. 2048 bits, not only the compiled
+ * ones): the role of the reference's runtime-precision MPFR / MPC device path
+ * (cu_mpfr_* / cu_mpc_*, demos/axpy_mpfr.cu:36-50, demos/axpy_mpc.cu:40-53),
+ * with MPFR's RNDN (MPC_RNDNN) results.  MP arrays of precision prec use
+ * ceil(prec/64) limbs, the significand left-justified (MPFR's layout, low
+ * bits zero).  Real zero signs follow MPFR (a product's zero has the sign of
+ * the product of the signs; -0 + -0 = -0, any other exact zero sum is +0); an
+ * exactly cancelling complex component is +0.
+ * mpgfx_rt_elementwise_real: o[i] = a[i] OP b[i], OP in {mul, add, sub}.
+ * mpgfx_rt_axpy_real: o[i] = RNDN(RNDN(a x[i]) + y[i]) (a: ONE value).
+ * mpgfx_rt_axpy_complex: z[i] = a x[i] + y[i], MPC_RNDNN per operation. */
+int mpgfx_rt_elementwise_real (int prec, int op, int64_t n,
+                               const uint64_t *a_limbs, const int64_t *a_se,
+                               const uint64_t *b_limbs, const int64_t *b_se,
+                               uint64_t *o_limbs, int64_t *o_se, void *stream);
+int mpgfx_rt_axpy_real (int prec, int64_t n,
+                        const uint64_t *a_limbs, const int64_t *a_se,
+                        const uint64_t *x_limbs, const int64_t *x_se,
+                        const uint64_t *y_limbs, const int64_t *y_se,
+                        uint64_t *o_limbs, int64_t *o_se, void *stream);
+int mpgfx_rt_axpy_complex (int prec, int64_t n,
+                           const uint64_t *a_limbs, const int64_t *a_se,
+                           const uint64_t *x_limbs, const int64_t *x_se,
+                           const uint64_t *y_limbs, const int64_t *y_se,
+                           uint64_t *z_limbs, int64_t *z_se, void *stream);
+
 /* n seeded synthetic nonzero values (value i = start + i of the stream keyed by
  * seed: significand limbs random with the top bit set and the low 64N-pb bits
  * clear, random sign, exponent uniform in [emin, emax]) -- the generator of the

@@ -23,6 +23,9 @@ SIGNATURES = {
     "mpgfx_elementwise_real": (_i, [_i, _i, _l] + [_p] * 6 + [_p]),
     "mpgfx_elementwise_complex": (_i, [_i, _i, _l] + [_p] * 6 + [_p]),
     "mpgfx_sqrt_real": (_i, [_i, _l] + [_p] * 4 + [_p]),
+    "mpgfx_rt_elementwise_real": (_i, [_i, _i, _l] + [_p] * 6 + [_p]),
+    "mpgfx_rt_axpy_real": (_i, [_i, _l] + [_p] * 8 + [_p]),
+    "mpgfx_rt_axpy_complex": (_i, [_i, _l] + [_p] * 8 + [_p]),
     "mpgfx_axpy_real": (_i, [_i, _l] + [_p] * 8 + [_p]),
     "mpgfx_axpy_complex": (_i, [_i, _l] + [_p] * 8 + [_p]),
     "mpgfx_gemv_real": (_i, [_i, _l, _l] + [_p] * 6 + [_p]),

@@ -3,6 +3,7 @@
  * point validates its arguments on the host and launches nothing on error. */
 #include "common.h"
 #include "dispatch.h"
+#include "rt.h"
 #include <cstring>
 
 namespace mpgfx {
@@ -227,4 +228,55 @@ extern "C" int mpgfx_fill_random (int pb, int64_t n, int64_t start, uint64_t see
   hipLaunchKernelGGL (mpgfx::k_fill_random, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, (hipStream_t) stream, n,
                       start, seed, nl, sb, emin, emax, limbs, se);
   return launch_status ("mpgfx_fill_random");
+}
+
+/* ---------------- run-time precision (rt.h, rt_kernels.hip) ---------------- */
+namespace {
+const mpgfx::rt_ops *rt_get (int p)
+{
+  const int nl = (p + 63) / 64;
+  if (nl <= 1) return mpgfx::rt_ops_cw_2 ();
+  if (nl <= 2) return mpgfx::rt_ops_cw_4 ();
+  if (nl <= 4) return mpgfx::rt_ops_cw_8 ();
+  if (nl <= 8) return mpgfx::rt_ops_cw_16 ();
+  if (nl <= 16) return mpgfx::rt_ops_cw_32 ();
+  return mpgfx::rt_ops_cw_64 ();
+}
+int rt_check (int prec, int64_t n)
+{
+  if (n < 0 || prec < 1) return MPGFX_EINVAL;
+  if (prec > mpgfx::MPGFX_RT_MAXPREC) return MPGFX_EUNSUPPORTED;
+  return MPGFX_OK;
+}
+} // namespace
+
+extern "C" int mpgfx_rt_elementwise_real (int prec, int op, int64_t n, const uint64_t *al, const int64_t *ase,
+                                          const uint64_t *bl, const int64_t *bse, uint64_t *ol, int64_t *ose,
+                                          void *stream)
+{
+  if (op < MPGFX_MUL || op > MPGFX_SUB) return MPGFX_EINVAL;
+  if (int rc = rt_check (prec, n)) return rc;
+  if (n == 0) return MPGFX_OK;
+  NEED (al && ase && bl && bse && ol && ose);
+  return rt_get (prec)->ew (op, prec, n, al, ase, bl, bse, ol, ose, (hipStream_t) stream);
+}
+
+extern "C" int mpgfx_rt_axpy_real (int prec, int64_t n, const uint64_t *al, const int64_t *ase, const uint64_t *xl,
+                                   const int64_t *xse, const uint64_t *yl, const int64_t *yse, uint64_t *ol,
+                                   int64_t *ose, void *stream)
+{
+  if (int rc = rt_check (prec, n)) return rc;
+  if (n == 0) return MPGFX_OK;
+  NEED (al && ase && xl && xse && yl && yse && ol && ose);
+  return rt_get (prec)->axpy_real (prec, n, al, ase, xl, xse, yl, yse, ol, ose, (hipStream_t) stream);
+}
+
+extern "C" int mpgfx_rt_axpy_complex (int prec, int64_t n, const uint64_t *al, const int64_t *ase,
+                                      const uint64_t *xl, const int64_t *xse, const uint64_t *yl,
+                                      const int64_t *yse, uint64_t *zl, int64_t *zse, void *stream)
+{
+  if (int rc = rt_check (prec, n)) return rc;
+  if (n == 0) return MPGFX_OK;
+  NEED (al && ase && xl && xse && yl && yse && zl && zse);
+  return rt_get (prec)->axpy_cplx (prec, n, al, ase, xl, xse, yl, yse, zl, zse, (hipStream_t) stream);
 }

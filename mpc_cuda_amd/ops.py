@@ -60,6 +60,32 @@ def sqrt(x, out=None):
     return out
 
 
+def rt_elementwise(op, a, b, out=None):
+    """Run-time precision (any a.pb in 1..2048): out = a OP b, OP in mul/add/sub,
+    RNDN at a.pb with MPFR's zero signs."""
+    _dev(a), _dev(b)
+    if a.pb != b.pb or a.n != b.n or a.cplx or b.cplx:
+        raise ValueError("shape mismatch")
+    out = out if out is not None else empty(a.pb, a.n, False, device=a.se.device)
+    L = _lib.lib()
+    _lib.check("mpgfx_rt_elementwise_real",
+               L.mpgfx_rt_elementwise_real(a.pb, _OPS[op], a.n, *a.ptrs(), *b.ptrs(), *out.ptrs(), _stream()))
+    return out
+
+
+def rt_axpy(a, x, y, out=None):
+    """Run-time precision AXPY (any x.pb in 1..2048), real or complex: RNDN /
+    MPC_RNDNN per operation, as mpfr_mul + mpfr_add / mpc_mul + mpc_add."""
+    _dev(a), _dev(x), _dev(y)
+    if not (a.pb == x.pb == y.pb) or a.n != 1 or x.n != y.n or not (a.cplx == x.cplx == y.cplx):
+        raise ValueError("shape mismatch")
+    out = out if out is not None else empty(x.pb, x.n, x.cplx, device=x.se.device)
+    L = _lib.lib()
+    f = L.mpgfx_rt_axpy_complex if x.cplx else L.mpgfx_rt_axpy_real
+    _lib.check(f.__name__, f(x.pb, x.n, *a.ptrs(), *x.ptrs(), *y.ptrs(), *out.ptrs(), _stream()))
+    return out
+
+
 def axpy(a, x, y, out=None):
     """out = RNDN(RNDN(a*x) + y) with a scalar (an MPArray of one value)."""
     _dev(a), _dev(x), _dev(y)

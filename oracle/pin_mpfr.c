@@ -120,7 +120,7 @@ int main (int argc, char **argv)
   if (argc < 6) die ("usage: pin_mpfr OP PB IN OUT M [N K]");
   const char *op = argv[1];
   pb = atoi (argv[2]); N = (pb + 63) / 64;
-  if (pb <= 0 || pb % 32 || N > 128) die ("bad PB");
+  if (pb <= 0 || N > 128) die ("bad PB");
   FILE *fi = fopen (argv[3], "rb"); if (!fi) die ("open in");
   FILE *fo = fopen (argv[4], "wb"); if (!fo) die ("open out");
   int64_t M = atoll (argv[5]);

@@ -52,6 +52,10 @@ def test_argument_errors_return_before_any_launch():
     assert L.mpgfx_sqrt_real(256, -1, p, p, p, p, None) == _lib.EINVAL
     assert L.mpgfx_sqrt_real(256, 8, p, None, p, p, None) == _lib.EINVAL
     assert L.mpgfx_sqrt_real(100, 8, p, p, p, p, None) == _lib.EUNSUPPORTED
+    assert L.mpgfx_rt_axpy_real(0, 8, p, p, p, p, p, p, p, p, None) == _lib.EINVAL
+    assert L.mpgfx_rt_axpy_complex(4096, 8, p, p, p, p, p, p, p, p, None) == _lib.EUNSUPPORTED
+    assert L.mpgfx_rt_elementwise_real(100, _lib.DIV, 8, p, p, p, p, p, p, None) == _lib.EINVAL
+    assert L.mpgfx_rt_axpy_real(100, 8, p, None, p, p, p, p, p, p, None) == _lib.EINVAL
     assert L.mpgfx_gemm_real(256, 5, 4, 4, 4, p, p, p, p, p, p, None, 0, None) == _lib.EINVAL
     # null pointers
     assert L.mpgfx_axpy_complex(256, 8, None, p, p, p, p, p, p, p, None) == _lib.EINVAL
