@@ -120,3 +120,28 @@ extern "C" int host_fmm_check (int pb, int64_t n, const uint64_t *l, const int64
 #undef C
   return -1;
 }
+
+/* RNDN(a*b) by the short product (with its `amb` flag) and by the exact one;
+ * operands stacked: limbs (2, N, n), se (2, n) */
+template <int PB> void fmul_chk (int64_t n, const uint64_t *l, const int64_t *se, uint64_t *sl, int64_t *sse,
+                                 uint64_t *el, int64_t *ese, uint8_t *amb)
+{
+  const int64_t o = (int64_t) cu_freal<PB>::N * n;
+#pragma omp parallel for
+  for (int64_t i = 0; i < n; i++)
+    {
+      cu_freal<PB> a = ld<PB> (l, se, n, i), b = ld<PB> (l + o, se + n, n, i);
+      bool f = false;
+      st<PB> (core::fmul_short<PB> (a, b, f), sl, sse, n, i);
+      st<PB> (core::fmul_exact<PB> (a, b), el, ese, n, i);
+      amb[i] = f ? 1 : 0;
+    }
+}
+extern "C" int host_fmul_check (int pb, int64_t n, const uint64_t *l, const int64_t *se, uint64_t *sl,
+                                int64_t *sse, uint64_t *el, int64_t *ese, uint8_t *amb)
+{
+#define C(P) if (pb == P) { fmul_chk<P> (n, l, se, sl, sse, el, ese, amb); return 0; }
+  PB_SHORT (C)
+#undef C
+  return -1;
+}

@@ -150,6 +150,36 @@ def test_complex_products_the_short_path_leaves_open(gpu, pb):
     assert same(M.axpy(ca.cuda(), x.cuda(), y.cuda()), O.axpy_complex(pb, ca.pair(), x.pair(), y.pair())) == 0
 
 
+@pytest.mark.parametrize("pb", [512, 1024, 1056, 2048])
+def test_real_products_near_a_midpoint(gpu, pb):
+    """Real products just above a rounding midpoint,
+    (2^(pb-1) + 2^(pb/2) + 1)(2^(pb-1) + 2^(pb/2-2) + 1) -- the case the short
+    product (used at pb 512) cannot decide from its kept columns -- mixed into
+    random ones, through the elementwise product, the AXPY and the GEMV;
+    oracle bit-exact."""
+    from mpcases import pack, unpack
+    from mpc_cuda_amd.arrays import nlimbs
+    top = 1 << (pb - 1)
+    v = lambda m, e, s=1: (s, e, m << (64 * nlimbs(pb) - pb), False)
+    n = 300
+    X, Y = unpack(M.random_real(pb, n, 40 + pb, -20, 20)), unpack(M.random_real(pb, n, 41 + pb, -20, 20))
+    for k in (0, 37, 150, 299):
+        X[k], Y[k] = v(top + (1 << (pb // 2)) + 1, 3, -1), v(top + (1 << (pb // 2 - 2)) + 1, -2)
+    x, y = pack(X, pb), pack(Y, pb)
+    assert same(M.elementwise("mul", x.cuda(), y.cuda()), O.elementwise_real(pb, "mul", x.pair(), y.pair())) == 0
+    a = pack([v(top + (1 << (pb // 2)) + 1, 1)], pb)
+    Xa = list(X)
+    for k in (5, 64, 200):
+        Xa[k] = v(top + (1 << (pb // 2 - 2)) + 1, 2)
+    xa = pack(Xa, pb)
+    assert same(M.axpy(a.cuda(), xa.cuda(), y.cuda()), O.axpy_real(pb, a.pair(), xa.pair(), y.pair())) == 0
+    m, k = 4, 75
+    A = pack(X[:m * k], pb)
+    xv = pack(Y[:k], pb)
+    g = M.gemv(A.cuda(), xv.cuda(), m, k)
+    assert same(g, O.gemm(pb, m, 1, k, A.pair(), xv.pair(), "perop")) == 0
+
+
 @pytest.mark.parametrize("pb", PBS)
 def test_gemm_gemv_perop_vs_golden_and_oracle(gpu, pb):
     g = goldens.load(pb)

@@ -1,6 +1,7 @@
-"""The short-product path of the complex product component RNDN(a*b -/+ c*d)
-(core::fmm_short in include/mpc_cuda/cu_fcomplex.cuh, used for the
-limb-product precisions) against the exact path (core::fmm_exact), both from
+"""The short-product paths -- of the complex product component RNDN(a*b -/+
+c*d) (core::fmm_short, include/mpc_cuda/cu_fcomplex.cuh) and of the real
+product RNDN(a*b) (core::fmul_short, cu_freal.cuh), used for the limb-product
+precisions -- against the exact paths (core::fmm_exact, core::fmul_exact), both from
 the g++ build of the headers (tests/host/libcufp_host.so): wherever the short
 path does not flag a lane as undecided, its result is the exact one, bit for
 bit; the cases its error bound cannot decide (exact and massive cancellation,
@@ -27,6 +28,7 @@ def host():
     L = ctypes.CDLL(HOSTLIB)
     L.host_fmm_check.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int] + \
         [ctypes.c_void_p] * 5
+    L.host_fmul_check.argtypes = [ctypes.c_int, ctypes.c_int64] + [ctypes.c_void_p] * 7
     return L
 
 
@@ -95,3 +97,35 @@ def test_midpoint_moved_by_dropped_bits_is_flagged(host, pb):
     dn = pack([v(top, -pb, -1)], pb)
     short, exact, amb = run(host, pb, [a, a, c, dn], 1)
     assert amb.all()
+
+
+def run_mul(L, pb, a, b):
+    n = a.n
+    N = nlimbs(pb)
+    l = np.ascontiguousarray(np.stack([a.limbs, b.limbs]))
+    s = np.ascontiguousarray(np.stack([a.se, b.se]))
+    sl, el = np.zeros((N, n), np.uint64), np.zeros((N, n), np.uint64)
+    ss, es = np.zeros(n, np.int64), np.zeros(n, np.int64)
+    amb = np.zeros(n, np.uint8)
+    assert L.host_fmul_check(pb, n, l.ctypes.data, s.ctypes.data, sl.ctypes.data, ss.ctypes.data,
+                             el.ctypes.data, es.ctypes.data, amb.ctypes.data) == 0
+    return (sl, ss), (el, es), amb.astype(bool)
+
+
+@pytest.mark.parametrize("pb", PBS)
+def test_short_real_product(host, pb):
+    """Random products: never flagged, always exact.  (2^(pb-1) + 2^(pb/2) + 1)
+    (2^(pb-1) + 2^(pb/2-2) + 1): the kept columns put the product exactly on a
+    midpoint with an even last bit (rounded down), the dropped low columns
+    lift it above (rounded up): flagged, and the two paths differ."""
+    a = G.random_real(pb, 3000, 77 + pb, -40, 40)
+    b = G.random_real(pb, 3000, 78 + pb, -40, 40)
+    short, exact, amb = run_mul(host, pb, a, b)
+    assert amb.sum() == 0 and agree_where_decided(short, exact, amb)
+    top = 1 << (pb - 1)
+    v = lambda m, e, s=1: (s, e, m << (64 * nlimbs(pb) - pb), False)
+    x = pack([v(top + (1 << (pb // 2)) + 1, 0)], pb)
+    y = pack([v(top + (1 << (pb // 2 - 2)) + 1, 0)], pb)
+    short, exact, amb = run_mul(host, pb, x, y)
+    assert amb.all()
+    assert not (np.array_equal(short[0], exact[0]) and np.array_equal(short[1], exact[1]))
