@@ -363,9 +363,10 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
 {
   const int L = LW + 2;
   rres<KW> r;
-  /* order by exponent (zero is the smallest); with equal exponents the
-   * difference may come out negative, and is negated below */
-  bool age = zb || (!za && ea >= eb);
+  /* order by exponent, then by the top words (zero is the smallest); only
+   * equal exponents and equal top words can leave the difference negative,
+   * and it is negated below */
+  bool age = zb || (!za && (ea > eb || (ea == eb && Ai[LW - 1] >= Bi[LW - 1])));
   u32 A[LW], B[LW];
   CUFP_UNROLL
   for (int i = 0; i < LW; i++)
