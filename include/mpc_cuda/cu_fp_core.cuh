@@ -138,6 +138,19 @@ CUFP_INL void mul_words_rows (u32 (&p)[2 * W], const u32 (&a)[W], const u32 (&b)
   });
 }
 
+/* bits [LB*k0 + s0, LB*k0 + s0 + 32) of a number held as LB-bit limbs P[0..NP)
+ * (s0 < LB): the word spans limbs k0, k0+1 and, when 2 LB - s0 < 32, k0+2;
+ * 32-bit shifts only */
+template <int LB, int NP, int k0, int s0>
+CUFP_INL u32 limbs_word (const u32 *P)
+{
+  static_assert (s0 >= 0 && s0 < LB, "offset inside the first limb");
+  u32 w = P[k0] >> s0;
+  if constexpr (k0 + 1 < NP) w |= P[k0 + 1] << (LB - s0);
+  if constexpr (k0 + 2 < NP && 2 * LB - s0 < 32) w |= P[k0 + 2] << (2 * LB - s0);
+  return w;
+}
+
 template <int W>
 struct limb_cut
 {
@@ -181,15 +194,10 @@ CUFP_INL void mul_words_limbs (u32 (&p)[2 * W], const u32 (&a)[W], const u32 (&b
     acc >>= LB;
   });
   P[2 * NL - 1] = (u32) acc;
-  /* limbs -> 32-bit words: word q = bits [32q, 32q+32), spanning <= 3 limbs */
+  /* limbs -> 32-bit words */
   sfor<0, 2 * W> ([&] (auto Q) {
     constexpr int q = decltype (Q)::value;
-    constexpr int k0 = (32 * q) / LB, s0 = 32 * q - LB * k0;
-    const u64 l0 = P[k0];
-    const u64 l1 = (k0 + 1 < 2 * NL) ? P[k0 + 1] : 0u;
-    const u64 l2 = (k0 + 2 < 2 * NL) ? P[k0 + 2] : 0u;
-    const u64 v = l0 | (l1 << LB) | (l2 << (2 * LB));
-    p[q] = (u32) (v >> s0);
+    p[q] = limbs_word<LB, 2 * NL, (32 * q) / LB, 32 * q - LB * ((32 * q) / LB)> (P);
   });
 }
 
@@ -240,11 +248,7 @@ CUFP_INL void mul_words_short (u32 (&p)[H], const u32 (&a)[W], const u32 (&b)[W]
     constexpr int q = 2 * W - H + decltype (Q)::value;
     constexpr int k0 = (32 * q) / LB, s0 = 32 * q - LB * k0;
     static_assert (k0 >= K0, "word below the computed columns");
-    const u64 l0 = P[k0];
-    const u64 l1 = (k0 + 1 < 2 * NL) ? P[k0 + 1] : 0u;
-    const u64 l2 = (k0 + 2 < 2 * NL) ? P[k0 + 2] : 0u;
-    const u64 v = l0 | (l1 << LB) | (l2 << (2 * LB));
-    p[decltype (Q)::value] = (u32) (v >> s0);
+    p[decltype (Q)::value] = limbs_word<LB, 2 * NL, k0, s0> (P);
   });
 }
 
