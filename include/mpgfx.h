@@ -175,6 +175,7 @@ int mpgfx_unpack (int pb, int complex, int64_t n, const uint64_t *limbs,
  * storage of cu_freal<PB> (left-justified, little-endian limbs, cu_freal.cuh:12-15)
  * and its exponent is cu_freal's `exp`, so each value copies as-is; nothing
  * is rounded.
+ * pb: a compiled precision or any precision of the run-time path (1 .. 2048).
  * import: every value must have precision pb and be zero or regular (NaN and
  * infinities give MPGFX_EINVAL); the limbs land in the device MP array.
  * export: every struct must have precision pb and a limb buffer of

@@ -40,7 +40,8 @@ int hip_ok (const char *where, hipError_t e)
   return MPGFX_OK;
 }
 
-bool pb_ok (int pb) { return pb > 0 && pb % 32 == 0 && mpgfx_supported_pb (pb); }
+/* the compiled precisions, and every precision of the run-time path */
+bool pb_ok (int pb) { return (pb >= 1 && pb <= 2048) || mpgfx_supported_pb (pb); }
 
 } // namespace
 

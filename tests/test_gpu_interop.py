@@ -138,3 +138,24 @@ def _filled(pb, h):
     for i in range(h.n):
         _fill(arr, i, h, i)
     return arr, bufs
+
+
+def test_import_runtime_precision_axpy_export():
+    """An mpfr_t array of precision 1000 (not compiled in) imports, goes
+    through the run-time precision AXPY, and exports; the exported structs
+    hold the oracle's values."""
+    import mpc_cuda_amd as M
+    from oracle import oracle as O
+    p, n = 1000, 77
+    a, x, y = G.random_real(p, 1, 3), G.random_real(p, n, 4, -30, 30), G.random_real(p, n, 5, -30, 30)
+    sa, ba = _filled(p, a)
+    sx, bx = _filled(p, x)
+    sy, by = _filled(p, y)
+    da, dx, dy = (import_mpfr(p, v.n, ctypes.addressof(s)) for v, s in ((a, sa), (x, sx), (y, sy)))
+    r = M.rt_axpy(da, dx, dy)
+    out, obufs = _structs(p, n)
+    export_mpfr(r, ctypes.addressof(out))
+    want = O.axpy_real(p, a.pair(), x.pair(), y.pair())
+    for i in range(n):
+        lw, s = _read(out, i, p)
+        assert s == int(want[1][i]) and lw == [int(v) for v in want[0][:, i]]
