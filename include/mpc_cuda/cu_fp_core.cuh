@@ -358,8 +358,8 @@ struct rres
  * less than 2^ERRB units of the last word of the larger one (together), and
  * r.amb reports whether the true sum could round differently: that is when
  * it lies within that error of a rounding midpoint, or the computed sum is 0.
- * With 4 guard words below the kept ones (LW = KW + 3) the test reads the
- * top 64 guard bits.  p > 0 rounds to p bits (a run-time precision, the
+ * With 3 or 4 guard words below the kept ones (LW = KW + 2 or KW + 3) the
+ * test reads the top 64 guard bits.  p > 0 rounds to p bits (a run-time precision, the
  * operands' low bits zero) rather than to KW words. */
 template <int KW, int LW, int ERRB = 0>
 CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
@@ -505,12 +505,13 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
     {
       /* E's last bit is worth 2^(cy - lz - 32) units of A's last word, so
        * the error is below 2^(ERRB + 32 - cy + lz) of E's last bits; the
-       * guard bits g = E[0..3] (round bit on top) are safe unless
-       * |g - 2^127| <= that, tested on the bits of E[3]:E[2] above it */
-      static_assert (L - 1 - KW == 4, "the test reads 4 guard words");
-      static_assert (ERRB >= 33, "error position below the tested guard bits");
-      const int t = ERRB + 32 - (int) cy + (int) lz - 64;
-      const u64 x = ((u64) E[3] << 32) | E[2];
+       * GW guard words g = E[0..GW-1] (round bit on top) are safe unless
+       * |g - 2^(32 GW - 1)| <= that, tested on the top two guard words */
+      constexpr int GW = L - 1 - KW;
+      static_assert (GW == 3 || GW == 4, "the test reads the top two of 3 or 4 guard words");
+      static_assert (ERRB + 31 - 32 * (GW - 2) >= 0, "error position below the tested guard bits");
+      const int t = ERRB + 32 - (int) cy + (int) lz - 32 * (GW - 2);
+      const u64 x = ((u64) E[GW - 1] << 32) | E[GW - 2];
       bool amb = true;
       if (t < 63)
         {
