@@ -54,12 +54,20 @@ CUFP_INL void sfor (F &&f)
     }
 }
 
-/* s = a + b + c, c <- carry out (c in {0,1}) */
+/* s = a + b + c, c <- carry out (c in {0,1}); on the device a 32-bit
+ * add-with-carry (v_addc_co_u32 through VCC) rather than a 64-bit add */
 CUFP_INL u32 addc (u32 a, u32 b, u32 &c)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+  u32 co;
+  const u32 s = __builtin_addc (a, b, c, &co);
+  c = co;
+  return s;
+#else
   u64 t = (u64) a + b + c;
   c = (u32) (t >> 32);
   return (u32) t;
+#endif
 }
 
 /* Funnel shifts.  On the device they are v_alignbit_b32; the 64-bit form is
