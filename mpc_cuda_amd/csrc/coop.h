@@ -164,18 +164,39 @@ __device__ __forceinline__ cval coop_add (const cval &a, const cval &b)
   return v;
 }
 
-/* frame part of a value stored as a record of N limbs then the sign/exponent
- * word (NL + 1 words); lanes 1..N hold limbs 0..N-1 */
+/* a record of N limbs then the sign/exponent word (NL + 1 words), as one
+ * lane loads it: limb j-1 (lanes outside 1..N load a limb of the same record
+ * and drop it in coop_decode) and the sign/exponent word.  Every lane loads,
+ * with no branch and no use of the values until coop_decode, so a prefetch
+ * stays in flight until the step that needs it. */
+struct craw
+{
+  u64 m;
+  i64 w;
+};
+
 template <int PB>
-__device__ __forceinline__ cval coop_load_rec (const u64 *__restrict__ rec)
+__device__ __forceinline__ craw coop_load_raw (const u64 *__restrict__ rec)
+{
+  constexpr int NL = (PB + 63) / 64, COOP_G = coop_lanes (PB);
+  const int j = (int) (threadIdx.x & (COOP_G - 1));
+  const int jj = j < 1 ? 0 : (j > NL ? NL - 1 : j - 1);
+  craw v;
+  v.m = rec[jj];
+  v.w = (i64) rec[NL];
+  return v;
+}
+
+/* frame part of a loaded record: lanes 1..N hold limbs 0..N-1 */
+template <int PB>
+__device__ __forceinline__ cval coop_decode (const craw &r)
 {
   constexpr int NL = (PB + 63) / 64, COOP_G = coop_lanes (PB);
   const int j = (int) (threadIdx.x & (COOP_G - 1));
   cval v;
-  v.m = (j >= 1 && j <= NL) ? rec[j - 1] : 0ull;
-  const i64 w = (i64) rec[NL];
-  v.sign = (w & 1) ? -1 : 1;
-  v.exp = w >> 1;
+  v.m = (j >= 1 && j <= NL) ? r.m : 0ull;
+  v.sign = (r.w & 1) ? -1 : 1;
+  v.exp = r.w >> 1;
   return v;
 }
 
@@ -202,18 +223,23 @@ __device__ __forceinline__ void st_rec (const cu_freal<PB> &v, u64 *__restrict__
 /* acc = +0; acc = RNDN(acc + p_s) for s < len, p_s the record at rec0 + s*step
  * (in records of NL+1 words); the wavefront runs max_len steps (every group
  * must take part in the cross-lane operations), a group past its own length
- * keeps its value.  Records are prefetched COOP_PD steps ahead. */
-constexpr int COOP_PD = 4;
+ * keeps its value.  Records are prefetched COOP_PD steps ahead.  The caller
+ * guarantees that the records at rec0 + s*step exist for every s < max_len
+ * (max_len is wave-uniform), also for a group with len < max_len: a group
+ * loads, and then ignores, the records past its own length, and a prefetch
+ * past max_len - 1 loads record max_len - 1 again. */
+constexpr int COOP_PD = 8;
 
 template <int PB>
 __device__ __forceinline__ cval coop_fold (const u64 *__restrict__ recs, i64 rec0, i64 step, i64 len, i64 max_len)
 {
   constexpr int NL = (PB + 63) / 64;
   cval acc = coop_zero ();
-  cval ring[COOP_PD];
+  if (max_len <= 0) return acc;
+  craw ring[COOP_PD];
 #pragma unroll
   for (int d = 0; d < COOP_PD; d++)
-    ring[d] = (d < len) ? coop_load_rec<PB> (recs + (rec0 + step * d) * (NL + 1)) : coop_zero ();
+    ring[d] = coop_load_raw<PB> (recs + (rec0 + step * (d < max_len ? d : max_len - 1)) * (NL + 1));
   for (i64 s0 = 0; s0 < max_len; s0 += COOP_PD)
     {
 #pragma unroll
@@ -222,12 +248,13 @@ __device__ __forceinline__ cval coop_fold (const u64 *__restrict__ recs, i64 rec
           const i64 s = s0 + d;
           if (s < max_len)
             {
-              const cval nv = coop_add<PB> (acc, ring[d]);
+              const cval nv = coop_add<PB> (acc, coop_decode<PB> (ring[d]));
               const bool live = s < len;
               acc.m = live ? nv.m : acc.m;
               acc.exp = live ? nv.exp : acc.exp;
               acc.sign = live ? nv.sign : acc.sign;
-              if (s + COOP_PD < len) ring[d] = coop_load_rec<PB> (recs + (rec0 + step * (s + COOP_PD)) * (NL + 1));
+              const i64 sn = s + COOP_PD < max_len ? s + COOP_PD : max_len - 1;
+              ring[d] = coop_load_raw<PB> (recs + (rec0 + step * sn) * (NL + 1));
             }
         }
     }
