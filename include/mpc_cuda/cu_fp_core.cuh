@@ -106,11 +106,11 @@ CUFP_INL bool any_lane (bool c)
 
 CUFP_INL int clz32 (u32 x)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
   return x ? __builtin_clz (x) : 32;
-#else
-  return x ? __builtin_clz (x) : 32;
-#endif
+}
+CUFP_INL int clz64 (u64 x)
+{
+  return x ? __builtin_clzll (x) : 64;
 }
 
 /* ---------------------------------------------------------------------------

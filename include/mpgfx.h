@@ -265,7 +265,11 @@ typedef struct mpgfx_batch mpgfx_batch;
  * the launches into a hipGraph on `stream`.  tuning: 0 = fixed default
  * mapping, 1 = the tuner's mapping (read from the JSON cache file `cache`,
  * or measured on this batch and written there when absent; cache may be NULL
- * to tune without a file).  The batch keeps its own copy of the descriptors. */
+ * to tune without a file).  The batch keeps its own copy of the descriptors.
+ * Aliasing: an AXPY may run in place (out == in2 or out == in1); no other
+ * output may overlap an input of the batch.  Tuning runs the batch's products
+ * on its inputs with the AXPY outputs sent to scratch, so create leaves every
+ * caller array as it found it apart from dot / GEMV outputs. */
 int mpgfx_batch_create (const mpgfx_task *tasks, int64_t ntasks, int tuning, const char *cache,
                         void *stream, mpgfx_batch **out);
 

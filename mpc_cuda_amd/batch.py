@@ -33,17 +33,34 @@ def _ptrs(x):
     return x.ptrs() if x is not None else (None, None)
 
 
+def _check(what, pb, cplx, *arrs):
+    """every array of a task: a device MP array of the task's precision and
+    kind, with its expected length (a mismatch would be an out-of-bounds
+    access on the device)"""
+    for x, n in arrs:
+        if not isinstance(x, MPArray) or not x.on_device:
+            raise TypeError(f"{what}: expected device MPArrays")
+        if x.pb != pb or x.cplx != cplx:
+            raise ValueError(f"{what}: precision / kind mismatch (pb={x.pb} cplx={x.cplx}, want pb={pb} cplx={cplx})")
+        if x.n != n:
+            raise ValueError(f"{what}: array of {x.n} values where {n} are needed")
+
+
 def axpy(a, x, y, out):
+    """out = a*x + y; out may be y itself (in place)"""
+    _check("batch.axpy", x.pb, x.cplx, (a, 1), (x, x.n), (y, x.n), (out, x.n))
     t = Task(AXPY, x.pb, int(x.cplx), 0, 1, x.n, *_ptrs(a), *_ptrs(x), *_ptrs(y), *_ptrs(out))
     return t, (a, x, y, out)
 
 
 def dot(x, y, out):
+    _check("batch.dot", x.pb, x.cplx, (x, x.n), (y, x.n), (out, 1))
     t = Task(DOT, x.pb, int(x.cplx), 0, 1, x.n, *_ptrs(x), *_ptrs(y), None, None, *_ptrs(out))
     return t, (x, y, out)
 
 
 def gemv(A, x, m, n, out):
+    _check("batch.gemv", x.pb, x.cplx, (A, m * n), (x, n), (out, m))
     t = Task(GEMV, x.pb, int(x.cplx), 0, m, n, *_ptrs(A), *_ptrs(x), None, None, *_ptrs(out))
     return t, (A, x, out)
 

@@ -292,6 +292,51 @@ int tune_folds (mpgfx_batch *b, hipStream_t s, std::map<std::string, mpgfx_mappi
   return MPGFX_OK;
 }
 
+/* While the tuner runs the batch's own products (many times), AXPY outputs go
+ * to one scratch buffer: an AXPY in place (out == in2, y = a*x + y) must still
+ * read its original y in the first real run.  on = false restores the task
+ * tables. */
+int redirect_axpy_outputs (mpgfx_batch *b, bool on, void **scratch)
+{
+  if (on)
+    {
+      size_t lw = 0, sw = 0;
+      for (auto &c : b->classes)
+        for (auto &t : c.tasks)
+          if (t.kind == MPGFX_TASK_AXPY)
+            {
+              const size_t parts = c.cplx ? 2 : 1;
+              lw = std::max (lw, (size_t) t.n * parts * (size_t) ((c.pb + 63) / 64));
+              sw = std::max (sw, (size_t) t.n * parts);
+            }
+      if (lw == 0) return MPGFX_OK;
+      int rc = hip_ok ("batch tuning outputs", hipMalloc (scratch, (lw + sw) * 8));
+      if (rc) return rc;
+      for (auto &c : b->classes)
+        {
+          std::vector<bt_task> tt = c.tasks;
+          for (auto &t : tt)
+            if (t.kind == MPGFX_TASK_AXPY)
+              {
+                t.outl = (u64 *) *scratch;
+                t.outs = (i64 *) ((u64 *) *scratch + lw);
+              }
+          rc = hip_ok ("batch tuning tasks", hipMemcpy (c.d_tasks, tt.data (), tt.size () * sizeof (bt_task),
+                                                        hipMemcpyHostToDevice));
+          if (rc) return rc;
+        }
+      return MPGFX_OK;
+    }
+  int rc = MPGFX_OK;
+  for (auto &c : b->classes)
+    if (!rc && !c.tasks.empty ())
+      rc = hip_ok ("batch tasks restore", hipMemcpy (c.d_tasks, c.tasks.data (), c.tasks.size () * sizeof (bt_task),
+                                                     hipMemcpyHostToDevice));
+  if (*scratch) (void) hipFree (*scratch);
+  *scratch = nullptr;
+  return rc;
+}
+
 void free_class (mpgfx_batch::cls &c)
 {
   void *ps[] = {c.d_tasks, c.d_items, c.d_chains, c.d_gbase, c.d_scr_l, c.d_scr_s, c.d_counters};
@@ -481,10 +526,15 @@ extern "C" int mpgfx_batch_create (const mpgfx_task *tasks, int64_t ntasks, int 
     {
       std::map<std::string, mpgfx_mapping> cmap = read_cache (cache);
       bool dirty = false;
+      void *tune_out = nullptr;
+      rc = redirect_axpy_outputs (b, true, &tune_out);
       for (auto &c : b->classes)
         if (!rc) rc = tune_class (c, b->cus, s, cmap, &dirty);
       if (!rc) rc = upload_classes ();
       if (!rc) rc = tune_folds (b, s, cmap, &dirty);
+      if (!rc) rc = hip_ok ("batch tuning sync", hipStreamSynchronize (s));
+      const int rrc = redirect_axpy_outputs (b, false, &tune_out);
+      rc = rc ? rc : rrc;
       if (!rc && dirty) write_cache (cache, cmap);
     }
   /* ---- graph: per class [reset -> products] on its own branch, then the

@@ -17,15 +17,19 @@
  *              pair is an INT8 GEMM on v_mfma_i32_16x16x64_i8 accumulating
  *              exactly in int32 (|digit product| <= 2^14, <= G*4096 terms per
  *              accumulator between flushes).  Digits are processed in groups of
- *              G = 16 or 20 (fewest passes): a pass keeps B's fragments in
- *              VGPRs, streams A's, and holds the 2G-1 diagonal accumulators in
- *              AGPRs; each pass flushes its diagonals, shifted by 8d
- *              bits, into per-element carry-save int64 slots (32-bit steps).
+ *              G in {9, 11, 13, 15, 17} (chosen by padded work and pass
+ *              overhead): a pass keeps B's fragments in VGPRs, streams A's,
+ *              and holds the 2G-1 diagonal accumulators; each pass flushes its
+ *              diagonals, shifted by 8d bits, into per-element carry-save
+ *              int64 slots (32-bit steps).  The slots hold any k below
+ *              2^62 / (G nG 2^38) (about 490,000 at pb 256); longer sums take
+ *              the fallback below.
  *  4. round    the slots are carry-normalised into a two's complement integer,
  *              negated if negative, and rounded once to pb bits (RNDN); the
  *              exponent is emaxA_i + emaxB_j - sA - sB - 2pb + bitlength.
- *  5. fallback spreads beyond the MFMA window (S > EXACT_SMAX digits) run on a
- *              VALU wide accumulator per output in workspace memory.
+ *  5. fallback spreads beyond the MFMA window (S > EXACT_SMAX digits), and k
+ *              beyond the slots' headroom, run on a VALU wide accumulator per
+ *              output in workspace memory (64 bits of headroom for the count).
  */
 #ifndef MPGFX_EXACT_H
 #define MPGFX_EXACT_H

@@ -267,6 +267,18 @@ static void choose_groups (int S, int *G, int *nG)
     }
 }
 
+/* Headroom of the int64 carry-save slots.  Per k term, a digit position D
+ * collects sum_{t+u=D} DA_t DB_u over at most spad digit pairs of magnitude
+ * <= 2^14, and a slot adds 4 digit positions shifted by 0, 8, 16 and 24 bits:
+ * at most spad * 2^14 * (1 + 2^8 + 2^16 + 2^24) per k.  Summed over kp terms
+ * that must stay below 2^62 (the slots are signed, and the rounding kernel's
+ * carry pass adds a carry); longer sums go to the wide-accumulator path. */
+static bool exact_slots_hold (int spad, i64 kp)
+{
+  const double per_k = (double) spad * 16384.0 * 16843009.0;
+  return (double) kp * per_k < 4.611686018427388e18;        /* 2^62 */
+}
+
 /* bytes of workspace for the MFMA path with digit count up to EXACT_SMAX */
 size_t exact_workspace_bound (int pb, i64 m, i64 n, i64 k)
 {
@@ -331,6 +343,10 @@ int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, con
   if (!p.fallback)
     {
       choose_groups (p.S, &p.G, &p.nG);
+      p.fallback = !exact_slots_hold (p.G * p.nG, kp);
+    }
+  if (!p.fallback)
+    {
       /* the MFMA kernel addresses one digit group with 32-bit offsets */
       if ((i64) p.G * (mp > np ? mp : np) * kp >= ((i64) 1 << 31)) return done (MPGFX_EUNSUPPORTED);
       const int spad = p.G * p.nG;

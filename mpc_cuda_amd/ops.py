@@ -29,6 +29,21 @@ def _same(pb, *xs):
             raise ValueError("precision mismatch")
 
 
+def _out(out, pb, n, cplx, like):
+    """a caller's output array, checked against the result's precision, length
+    and kind (a short or mismatched array would be written out of bounds), or
+    a new device array"""
+    if out is None:
+        return empty(pb, n, cplx, device=like.se.device)
+    _dev(out)
+    if out.pb != pb or out.n != n or out.cplx != cplx:
+        raise ValueError(f"output array is pb={out.pb} n={out.n} cplx={out.cplx}, "
+                         f"the result is pb={pb} n={n} cplx={cplx}")
+    if not (out.limbs.is_contiguous() and out.se.is_contiguous()):
+        raise ValueError("output array must be contiguous")
+    return out
+
+
 _OPS = {"mul": _lib.MUL, "add": _lib.ADD, "sub": _lib.SUB, "div": _lib.DIV}
 
 
@@ -41,7 +56,7 @@ def elementwise(op, a, b, out=None):
         raise ValueError("shape mismatch")
     if a.cplx and op == "div":
         raise ValueError("complex division is not provided")
-    out = out if out is not None else empty(a.pb, a.n, a.cplx, device=a.se.device)
+    out = _out(out, a.pb, a.n, a.cplx, a)
     L = _lib.lib()
     f = L.mpgfx_elementwise_complex if a.cplx else L.mpgfx_elementwise_real
     _lib.check(f.__name__, f(a.pb, _OPS[op], a.n, *a.ptrs(), *b.ptrs(), *out.ptrs(), _stream()))
@@ -54,7 +69,7 @@ def sqrt(x, out=None):
     _dev(x)
     if x.cplx:
         raise ValueError("complex square root is not provided")
-    out = out if out is not None else empty(x.pb, x.n, False, device=x.se.device)
+    out = _out(out, x.pb, x.n, False, x)
     L = _lib.lib()
     _lib.check("mpgfx_sqrt_real", L.mpgfx_sqrt_real(x.pb, x.n, *x.ptrs(), *out.ptrs(), _stream()))
     return out
@@ -66,7 +81,7 @@ def rt_elementwise(op, a, b, out=None):
     _dev(a), _dev(b)
     if a.pb != b.pb or a.n != b.n or a.cplx or b.cplx:
         raise ValueError("shape mismatch")
-    out = out if out is not None else empty(a.pb, a.n, False, device=a.se.device)
+    out = _out(out, a.pb, a.n, False, a)
     L = _lib.lib()
     _lib.check("mpgfx_rt_elementwise_real",
                L.mpgfx_rt_elementwise_real(a.pb, _OPS[op], a.n, *a.ptrs(), *b.ptrs(), *out.ptrs(), _stream()))
@@ -79,7 +94,7 @@ def rt_axpy(a, x, y, out=None):
     _dev(a), _dev(x), _dev(y)
     if not (a.pb == x.pb == y.pb) or a.n != 1 or x.n != y.n or not (a.cplx == x.cplx == y.cplx):
         raise ValueError("shape mismatch")
-    out = out if out is not None else empty(x.pb, x.n, x.cplx, device=x.se.device)
+    out = _out(out, x.pb, x.n, x.cplx, x)
     L = _lib.lib()
     f = L.mpgfx_rt_axpy_complex if x.cplx else L.mpgfx_rt_axpy_real
     _lib.check(f.__name__, f(x.pb, x.n, *a.ptrs(), *x.ptrs(), *y.ptrs(), *out.ptrs(), _stream()))
@@ -92,7 +107,7 @@ def axpy(a, x, y, out=None):
     _same(x.pb, a, y)
     if a.n != 1 or x.n != y.n or not (a.cplx == x.cplx == y.cplx):
         raise ValueError("shape mismatch")
-    out = out if out is not None else empty(x.pb, x.n, x.cplx, device=x.se.device)
+    out = _out(out, x.pb, x.n, x.cplx, x)
     L = _lib.lib()
     f = L.mpgfx_axpy_complex if x.cplx else L.mpgfx_axpy_real
     _lib.check(f.__name__, f(x.pb, x.n, *a.ptrs(), *x.ptrs(), *y.ptrs(), *out.ptrs(), _stream()))
@@ -105,7 +120,7 @@ def gemv(A, x, m, k, out=None):
     _same(A.pb, x)
     if A.n != m * k or x.n != k or A.cplx or x.cplx:
         raise ValueError("shape mismatch")
-    out = out if out is not None else empty(A.pb, m, False, device=A.se.device)
+    out = _out(out, A.pb, m, False, A)
     L = _lib.lib()
     _lib.check("mpgfx_gemv_real", L.mpgfx_gemv_real(A.pb, m, k, *A.ptrs(), *x.ptrs(), *out.ptrs(), _stream()))
     return out
@@ -119,7 +134,7 @@ def gemm(A, B, m, n, k, mode="perop", out=None, workspace=None):
     _same(A.pb, B)
     if A.n != m * k or B.n != k * n or A.cplx != B.cplx:
         raise ValueError("shape mismatch")
-    out = out if out is not None else empty(A.pb, m * n, A.cplx, device=A.se.device)
+    out = _out(out, A.pb, m * n, A.cplx, A)
     L = _lib.lib()
     if A.cplx:
         if mode == "perop":

@@ -105,3 +105,46 @@ def test_device_generator_matches_host(gpu, pb, cplx):
     d = M.device_random(pb, n, 99, -30, 30, cplx)
     h = M.random_array(pb, n, 99, -30, 30, cplx)
     assert M.equal(d, h) == 0
+
+
+@pytest.mark.parametrize("tuning", [0, 1])
+def test_batch_inplace_axpy(gpu, tuning, tmp_path):
+    """y = a*x + y in place (out is y) next to a dot and a GEMV: the tuner runs
+    the batch's products many times while it measures, yet the first real run
+    must see the original y; a second run applies the AXPY again."""
+    import torch
+    pb, n = 256, 5000
+    a, x = M.random_real(pb, 1, 1), M.random_real(pb, n, 2)
+    y0 = M.random_real(pb, n, 3)
+    y = y0.cuda()
+    u, v = M.random_complex(512, 3000, 4), M.random_complex(512, 3000, 5)
+    dout = M.empty(512, 1, True)
+    A, xv = M.random_real(128, 40 * 700, 6), M.random_real(128, 700, 7)
+    gout = M.empty(128, 40)
+    tasks = [B.axpy(a.cuda(), x.cuda(), y, y), B.dot(u.cuda(), v.cuda(), dout),
+             B.gemv(A.cuda(), xv.cuda(), 40, 700, gout)]
+    b = B.Batch(tasks, tuning=bool(tuning), cache=str(tmp_path / "t.json") if tuning else None)
+    b.run()
+    torch.cuda.synchronize()
+    r1 = O.axpy_real(pb, a.pair(), x.pair(), y0.pair())
+    assert O.mismatches(y.pair(), r1, ignore_zero_sign=False) == 0
+    assert O.mismatches(dout.pair(), O.gemm(512, 1, 1, 3000, u.pair(), v.pair(), "cperop"), ignore_zero_sign=False) == 0
+    assert O.mismatches(gout.pair(), O.gemm(128, 40, 1, 700, A.pair(), xv.pair(), "perop"), ignore_zero_sign=False) == 0
+    b.run()
+    torch.cuda.synchronize()
+    r2 = O.axpy_real(pb, a.pair(), x.pair(), r1)
+    assert O.mismatches(y.pair(), r2, ignore_zero_sign=False) == 0
+    b.close()
+
+
+def test_batch_task_shape_checks(gpu):
+    """task wrappers reject arrays of the wrong length, precision or kind"""
+    x, y = M.random_real(256, 10, 1).cuda(), M.random_real(256, 10, 2).cuda()
+    with pytest.raises(ValueError):
+        B.dot(x, M.random_real(256, 9, 3).cuda(), M.empty(256, 1))
+    with pytest.raises(ValueError):
+        B.axpy(M.random_real(256, 1, 4).cuda(), x, y, M.empty(256, 11))
+    with pytest.raises(ValueError):
+        B.gemv(M.random_real(256, 20, 5).cuda(), x, 3, 10, M.empty(256, 3))
+    with pytest.raises(ValueError):
+        B.dot(x, M.random_real(128, 10, 6).cuda(), M.empty(256, 1))

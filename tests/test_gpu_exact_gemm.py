@@ -132,3 +132,22 @@ def test_complex_exact_gemm_cancellation(gpu):
         b = (np.ascontiguousarray(Bl), np.ascontiguousarray(Bs))
         r = M.gemm(dev(a, pb, True), dev(b, pb, True), m, n, k, "exact")
         assert O.mismatches(r.pair(), O.gemm(pb, m, n, k, a, b, "cexact"), ignore_zero_sign=False) == 0, flip
+
+
+@pytest.mark.parametrize("k", [480000, 600000], ids=["mfma_at_limit", "beyond_slot_headroom"])
+def test_exact_gemm_long_k_constant_digits(gpu, k):
+    """Constant significands whose bytes are 0x7F (the largest positive digit,
+    127, in every digit position) and equal exponents put the most into every
+    carry-save slot per k term.  Up to about 490,000 terms (pb 256) the MFMA
+    path's int64 slots hold the sum; beyond, the call takes the wide
+    accumulator.  Closed form: C = RNDN(k * a * b)."""
+    import bigint_ref as R
+    pb, N = 256, 4
+    Mv = int("80" + "7f" * 31, 16)                     # top bit set, then 0x7F bytes
+    limbs = np.array([(Mv >> (64 * l)) & ((1 << 64) - 1) for l in range(N)], np.uint64)
+    A = MPArray(pb, np.ascontiguousarray(np.repeat(limbs[:, None], k, 1)), np.full(k, 3 * 2, np.int64))
+    B = MPArray(pb, np.ascontiguousarray(np.repeat(limbs[:, None], k, 1)), np.full(k, 5 * 2 + 1, np.int64))
+    r = M.gemm(A.cuda(), B.cuda(), 1, 1, k, "exact")
+    want = R.round_exact(-k * Mv * Mv, 3 + 5 - 2 * 64 * N, pb)
+    got = M.arrays.decode(r.pair()[0][:, 0], r.pair()[1][0], pb)
+    assert (got[0], got[1], got[2]) == (want[0], want[1], want[2])
