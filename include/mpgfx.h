@@ -137,6 +137,24 @@ int mpgfx_gemm_real (int pb, int mode, int64_t m, int64_t n, int64_t k,
                      uint64_t *C_limbs, int64_t *C_se,
                      void *ws, size_t ws_bytes, void *stream);
 
+/* The exact mode of mpgfx_gemm_real (C = RNDN(sum_k A[i,k]*B[k,j]), one
+ * rounding per output) without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (serving loops,
+ * harnesses that forbid hipMalloc).  Its workspace is a static device array
+ * of 256 MiB per GPU, held from the library's first use on that GPU until the
+ * process ends; the results are those of mpgfx_gemm_real(MPGFX_GEMM_EXACT).
+ * A call whose workspace would exceed the array returns MPGFX_ENOMEM with the
+ * sizes in mpgfx_last_error() (the size depends on the exponent spread, so
+ * the check follows the exponent scan, which writes only the array); pb 256,
+ * m = n = k = 1024 with exponent spreads below 8 needs 234 MB.  Like the exact
+ * mode it synchronises `stream` once.  Every call on one GPU uses the same
+ * array: calls that overlap on one GPU (other host threads, other streams)
+ * race and must be serialised by the caller. */
+int mpgfx_gemm_exact_noalloc (int pb, int64_t m, int64_t n, int64_t k,
+                              const uint64_t *A_limbs, const int64_t *A_se,
+                              const uint64_t *B_limbs, const int64_t *B_se,
+                              uint64_t *C_limbs, int64_t *C_se, void *stream);
+
 /* Complex per-operation GEMM (C = A*B, acc = +0, acc = acc + A[i,k]*B[k,j] with
  * MPC_RNDNN products and sums, k ascending).  Counterpart of the reference's
  * complex matmul demo order (demos/matmul_mpc.cu:33-52) on the fixed type. */

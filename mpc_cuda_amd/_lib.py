@@ -33,6 +33,7 @@ SIGNATURES = {
     "mpgfx_gemm_real": (_i, [_i, _i, _l, _l, _l] + [_p] * 6 + [_p, ctypes.c_size_t, _p]),
     "mpgfx_gemm_complex": (_i, [_i, _l, _l, _l] + [_p] * 6 + [_p]),
     "mpgfx_gemm_complex_exact": (_i, [_i, _l, _l, _l] + [_p] * 6 + [_p]),
+    "mpgfx_gemm_exact_noalloc": (_i, [_i, _l, _l, _l] + [_p] * 6 + [_p]),
     "mpgfx_import_mpfr": (_i, [_i, _l, _p, ctypes.c_size_t, _p, _p, _p]),
     "mpgfx_export_mpfr": (_i, [_i, _l, _p, _p, _p, ctypes.c_size_t, _p]),
     "mpgfx_pack": (_i, [_i, _i, _l, _p, _p, _p, _p]),

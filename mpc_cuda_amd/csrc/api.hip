@@ -15,6 +15,11 @@ void set_error (const char *where, hipError_t e)
   snprintf (g_err, sizeof g_err, "%s: %s", where, hipGetErrorString (e));
 }
 
+void set_error_text (const char *msg)
+{
+  snprintf (g_err, sizeof g_err, "%s", msg);
+}
+
 int launch_status (const char *where)
 {
   hipError_t e = hipGetLastError ();
@@ -128,8 +133,23 @@ extern "C" int mpgfx_gemm_real (int pb, int mode, int64_t m, int64_t n, int64_t 
   NEED (Cl && Cse);
   if (k > 0) NEED (Al && Ase && Bl && Bse);
   if (mode == MPGFX_GEMM_EXACT)
-    return gemm_exact (o, pb, m, n, k, Al, Ase, Bl, Bse, Cl, Cse, ws, ws_bytes, (hipStream_t) stream);
+    return gemm_exact (o, pb, m, n, k, Al, Ase, Bl, Bse, Cl, Cse, ws, ws_bytes, (hipStream_t) stream, false);
   return o->gemm_perop (m, n, k, Al, Ase, Bl, Bse, Cl, Cse, (hipStream_t) stream);
+}
+
+extern "C" int mpgfx_gemm_exact_noalloc (int pb, int64_t m, int64_t n, int64_t k, const uint64_t *Al,
+                                         const int64_t *Ase, const uint64_t *Bl, const int64_t *Bse, uint64_t *Cl,
+                                         int64_t *Cse, void *stream)
+{
+  if (m < 0 || n < 0 || k < 0) return MPGFX_EINVAL;
+  GET_OPS (o);
+  if (m == 0 || n == 0) return MPGFX_OK;
+  NEED (Cl && Cse);
+  if (k > 0) NEED (Al && Ase && Bl && Bse);
+  void *scr = nullptr;
+  const int rc = exact_noalloc_scratch (&scr);
+  if (rc) return rc;
+  return gemm_exact (o, pb, m, n, k, Al, Ase, Bl, Bse, Cl, Cse, scr, EXACT_NOALLOC_BYTES, (hipStream_t) stream, true);
 }
 
 extern "C" int mpgfx_gemm_complex (int pb, int64_t m, int64_t n, int64_t k, const uint64_t *Al, const int64_t *Ase,

@@ -58,6 +58,7 @@ __device__ __forceinline__ void st (const cu_freal<PB> &r, u64 *__restrict__ l, 
 
 /* host-side error bookkeeping */
 void set_error (const char *where, hipError_t e);
+void set_error_text (const char *msg);
 int launch_status (const char *where);
 
 inline int nlimbs (int pb) { return (pb + 63) / 64; }

@@ -52,7 +52,9 @@ int batch_folds_launch (const bt_class *cls, const bt_unit *units, i64 nunits, u
 /* exact-dot GEMM (exact_gemm.hip) */
 size_t exact_workspace_bound (int pb, i64 m, i64 n, i64 k);
 int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl,
-                const i64 *Bse, u64 *Cl, i64 *Cse, void *ws, size_t ws_bytes, hipStream_t s);
+                const i64 *Bse, u64 *Cl, i64 *Cse, void *ws, size_t ws_bytes, hipStream_t s, bool noalloc);
+/* device address of this GPU's static scratch of the no-allocation entry point */
+int exact_noalloc_scratch (void **out);
 int gemm_exact_complex (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl,
                         const i64 *Bse, u64 *Cl, i64 *Cse, hipStream_t s);
 

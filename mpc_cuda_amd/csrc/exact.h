@@ -42,6 +42,9 @@ constexpr int EXACT_SMAX = 80;          /* digits per operand on the MFMA path *
 constexpr int EXACT_SPAD_MAX = 96;      /* digits incl. the zero planes of the last group */
 constexpr i64 EXACT_EMPTY = INT64_MIN;  /* row/column without nonzero entries */
 constexpr int EXACT_KCHUNK = 4096;      /* int32 flush period in k (G <= 20) */
+/* static scratch of mpgfx_gemm_exact_noalloc: 256 MiB per GPU (pb 256,
+ * 1024^3 with exponent spreads below 8 needs 234 MB) */
+constexpr size_t EXACT_NOALLOC_BYTES = (size_t) 256 << 20;
 
 inline int exact_digits (int pb, i64 spread) { return (int) ((pb + spread + 2 + 7) / 8); }
 inline i64 pad_to (i64 x, i64 m) { return (x + m - 1) / m * m; }

@@ -2,10 +2,14 @@
  * the exact-dot GEMM (scheme in exact.h; per-precision kernels in
  * exact_pb.h).  The exponent scan has to reach the host before the digit count
  * and the MFMA kernel variant are chosen, so this mode synchronises its stream
- * once, after the scan. */
+ * once, after the scan.  Also here: the static scratch of the no-allocation
+ * entry point (mpgfx_gemm_exact_noalloc). */
 #include "common.h"
 #include "dispatch.h"
 #include "exact.h"
+
+#include <algorithm>
+#include <mutex>
 
 namespace mpgfx {
 
@@ -294,46 +298,67 @@ size_t exact_workspace_bound (int pb, i64 m, i64 n, i64 k)
   return b;
 }
 
+/* two zero words (a kernel rather than a memset, so the call can be captured) */
+__global__ void k_exact_zero2 (unsigned long long *p)
+{
+  if (threadIdx.x < 2) p[threadIdx.x] = 0ull;
+}
+
+/* bytes the MFMA path needs after the scan buffer: digit planes of A and B,
+ * then the carry-save slots */
+static size_t exact_mfma_bytes (i64 mp, i64 np, i64 kp, int spad)
+{
+  size_t b = (size_t) pad_to ((i64) spad * mp * kp, 256);
+  b += (size_t) pad_to ((i64) spad * np * kp, 256);
+  b += (size_t) mp * np * exact_slots (spad) * 8;
+  return b;
+}
+
+/* The workspace is the caller's `ws` when it is large enough for this call;
+ * what it lacks is allocated stream-ordered, or, with noalloc, refused
+ * (MPGFX_ENOMEM, before any launch for the scan buffer, after the scan for
+ * the rest: the digit count depends on the exponents). */
 int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl,
-                const i64 *Bse, u64 *Cl, i64 *Cse, void *ws, size_t ws_bytes, hipStream_t s)
+                const i64 *Bse, u64 *Cl, i64 *Cse, void *ws, size_t ws_bytes, hipStream_t s, bool noalloc)
 {
   if (k == 0)                      /* empty sums: +0 everywhere, as the per-op path */
     return o->gemm_perop (m, n, k, Al, Ase, Bl, Bse, Cl, Cse, s);
   const i64 mp = pad_to (m, 32), np = pad_to (n, 32), kp = pad_to (k, 64);
-  /* ---- scan (always needs a small buffer) ---- */
-  size_t scan_bytes = (size_t) pad_to ((i64) ((2 * m + 2 * n + 8) * 8), 256);
-  char *base = (char *) ws;
-  char *owned = nullptr;
-  size_t need_mfma = exact_workspace_bound (pb, m, n, k);
-  if (!base || ws_bytes < need_mfma)
+  const size_t scan_bytes = (size_t) pad_to ((i64) ((2 * m + 2 * n + 8) * 8), 256);
+  char *base = ws ? (char *) ws : nullptr;
+  size_t avail = ws ? ws_bytes : 0;
+  char *owned = nullptr, *owned2 = nullptr;
+  auto refuse = [&] (const char *what, size_t need) {
+    char msg[192];
+    snprintf (msg, sizeof msg, "gemm_exact: the %s needs %zu bytes of workspace, %zu available", what, need, avail);
+    set_error_text (msg);
+    return (int) MPGFX_ENOMEM;
+  };
+  if (avail < scan_bytes)
     {
-      int rc = hip_ok ("gemm_exact workspace", hipMallocAsync ((void **) &owned, need_mfma, s));
-      if (rc) return MPGFX_ENOMEM;
+      if (noalloc) return refuse ("exponent scan", scan_bytes);
+      if (hip_ok ("gemm_exact workspace", hipMallocAsync ((void **) &owned, scan_bytes, s))) return MPGFX_ENOMEM;
       base = owned;
+      avail = scan_bytes;
     }
   i64 *emaxA = (i64 *) base, *eminA = emaxA + m, *emaxB = eminA + m, *eminB = emaxB + n;
   unsigned long long *spreads = (unsigned long long *) (eminB + n);
   int rc = MPGFX_OK;
   auto done = [&] (int r) {
+    if (owned2) (void) hipFreeAsync (owned2, s);
     if (owned) (void) hipFreeAsync (owned, s);
     return r;
   };
-  if ((rc = hip_ok ("gemm_exact memset", hipMemsetAsync (spreads, 0, 2 * sizeof (unsigned long long), s)))) return done (rc);
-  if (k > 0)
-    {
-      hipLaunchKernelGGL (k_exact_scan, dim3 ((unsigned) m), dim3 (256), 0, s, m, k, k, (i64) 1, Ase, emaxA, eminA,
-                          spreads);
-      hipLaunchKernelGGL (k_exact_scan, dim3 ((unsigned) n), dim3 (256), 0, s, n, k, (i64) 1, n, Bse, emaxB, eminB,
-                          spreads + 1);
-      if ((rc = launch_status ("gemm_exact scan"))) return done (rc);
-    }
+  hipLaunchKernelGGL (k_exact_zero2, dim3 (1), dim3 (64), 0, s, spreads);
+  hipLaunchKernelGGL (k_exact_scan, dim3 ((unsigned) m), dim3 (256), 0, s, m, k, k, (i64) 1, Ase, emaxA, eminA,
+                      spreads);
+  hipLaunchKernelGGL (k_exact_scan, dim3 ((unsigned) n), dim3 (256), 0, s, n, k, (i64) 1, n, Bse, emaxB, eminB,
+                      spreads + 1);
+  if ((rc = launch_status ("gemm_exact scan"))) return done (rc);
   unsigned long long hs[2] = {0, 0};
-  if (k > 0)
-    {
-      if ((rc = hip_ok ("gemm_exact spread copy", hipMemcpyAsync (hs, spreads, sizeof hs, hipMemcpyDeviceToHost, s))))
-        return done (rc);
-      if ((rc = hip_ok ("gemm_exact sync", hipStreamSynchronize (s)))) return done (rc);
-    }
+  if ((rc = hip_ok ("gemm_exact spread copy", hipMemcpyAsync (hs, spreads, sizeof hs, hipMemcpyDeviceToHost, s))))
+    return done (rc);
+  if ((rc = hip_ok ("gemm_exact sync", hipStreamSynchronize (s)))) return done (rc);
   exact_plan p;
   p.pb = pb; p.m = m; p.n = n; p.k = k; p.mp = mp; p.np = np; p.kp = kp;
   p.sA = (i64) hs[0]; p.sB = (i64) hs[1];
@@ -351,8 +376,15 @@ int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, con
       if ((i64) p.G * (mp > np ? mp : np) * kp >= ((i64) 1 << 31)) return done (MPGFX_EUNSUPPORTED);
       const int spad = p.G * p.nG;
       p.slots = exact_slots (spad);
-      const int grid = (int) exact_grid ((mp / 32) * (np / 32));
+      const size_t need = exact_mfma_bytes (mp, np, kp, spad);
       char *q = base + scan_bytes;
+      if (avail < scan_bytes + need)
+        {
+          if (noalloc) return done (refuse ("digit planes and slots", scan_bytes + need));
+          if (hip_ok ("gemm_exact workspace", hipMallocAsync ((void **) &owned2, need, s))) return done (MPGFX_ENOMEM);
+          q = owned2;
+        }
+      const int grid = (int) exact_grid ((mp / 32) * (np / 32));
       int8_t *DA = (int8_t *) q;
       q += pad_to ((i64) spad * mp * kp, 256);
       int8_t *DB = (int8_t *) q;
@@ -368,20 +400,59 @@ int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, con
   const i64 spanbits = p.sA + p.sB + 2 * (i64) pb + 64 + 2;
   if (spanbits > ((i64) 1 << 22)) return done (MPGFX_EUNSUPPORTED);
   const int nw = (int) ((spanbits + 31) / 32) + 1;
+  const size_t row_bytes = (size_t) n * nw * 8;
   const size_t budget = (size_t) 256 << 20;
-  i64 rows = (i64) (budget / ((size_t) n * nw * 8));
-  if (rows < 1) rows = 1;
-  if (rows > m) rows = m;
   i64 *wsp = nullptr;
-  if ((rc = hip_ok ("gemm_exact fallback workspace", hipMallocAsync ((void **) &wsp, (size_t) rows * n * nw * 8, s))))
-    return done (MPGFX_ENOMEM);
+  i64 rows;
+  if (avail >= scan_bytes + row_bytes)
+    {
+      const size_t room = std::min (avail - scan_bytes, budget > row_bytes ? budget : row_bytes);
+      rows = (i64) (room / row_bytes);
+      wsp = (i64 *) (base + scan_bytes);
+    }
+  else
+    {
+      if (noalloc) return done (refuse ("wide accumulator", scan_bytes + row_bytes));
+      rows = (i64) (budget / row_bytes);
+      if (rows < 1) rows = 1;
+      if (rows > m) rows = m;
+      if ((rc = hip_ok ("gemm_exact fallback workspace", hipMallocAsync ((void **) &owned2, (size_t) rows * row_bytes, s))))
+        return done (MPGFX_ENOMEM);
+      wsp = (i64 *) owned2;
+    }
+  if (rows > m) rows = m;
   for (i64 r0 = 0; r0 < m && rc == MPGFX_OK; r0 += rows)
     {
       i64 nr = (r0 + rows <= m) ? rows : m - r0;
       rc = o->exact_fallback (m, n, k, r0, nr, Al, Ase, Bl, Bse, eminA, eminB, nw, wsp, Cl, Cse, s);
     }
-  (void) hipFreeAsync (wsp, s);
   return done (rc);
+}
+
+/* ---- the entry point that allocates nothing: its workspace is a static
+ * device array, one per GPU for the life of the process ---- */
+static __device__ __align__ (256) unsigned char scratch[EXACT_NOALLOC_BYTES];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[256];
+
+/* device address of this GPU's copy of `scratch`, looked up once per device
+ * (the host-side name of the array is not a device address) */
+int exact_noalloc_scratch (void **out)
+{
+  int dev = 0;
+  if (hip_ok ("gemm_exact_noalloc device", hipGetDevice (&dev))) return MPGFX_EHIP;
+  if (dev < 0 || dev >= 256) return MPGFX_EINVAL;
+  std::lock_guard<std::mutex> lock (g_scratch_mu);
+  if (!g_scratch_ptr[dev])
+    {
+      void *p = nullptr;
+      if (hip_ok ("gemm_exact_noalloc scratch", hipGetSymbolAddress (&p, HIP_SYMBOL (scratch)))) return MPGFX_EHIP;
+      if (!p) return MPGFX_EHIP;
+      g_scratch_ptr[dev] = p;
+    }
+  *out = g_scratch_ptr[dev];
+  return MPGFX_OK;
 }
 
 /* sign flip of every value of a real MP array (se ^= 1; zeros are skipped by
@@ -461,8 +532,8 @@ int gemm_exact_complex (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 
       hipLaunchKernelGGL (k_negate, dim3 ((unsigned) (g < 65536 ? g : 65536)), dim3 (256), 0, s, nB, Bis0, Brs + nB);
       rc = launch_status ("gemm_exact complex negate");
     }
-  if (!rc) rc = gemm_exact (o, pb, m, n, 2 * k, A2l, A2s, Brl, Brs, Cl, Cse, nullptr, 0, s);
-  if (!rc) rc = gemm_exact (o, pb, m, n, 2 * k, A2l, A2s, Bil, Bis, Cil, Cis, nullptr, 0, s);
+  if (!rc) rc = gemm_exact (o, pb, m, n, 2 * k, A2l, A2s, Brl, Brs, Cl, Cse, nullptr, 0, s, false);
+  if (!rc) rc = gemm_exact (o, pb, m, n, 2 * k, A2l, A2s, Bil, Bis, Cil, Cis, nullptr, 0, s, false);
   (void) hipFreeAsync (buf, s);
   return rc;
 }
