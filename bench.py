@@ -1,25 +1,41 @@
-"""Headline benchmark: multiple-precision ops/s of the fixed-precision hot path
-on one MI355X.
+"""Benchmark of the fixed-precision hot path on one MI355X.
 
-Workload (the configuration the metric is quoted on): complex AXPY
+Headline (the configuration the metric is quoted on): complex AXPY
 z = a*x + y in cu_fcomplex<1024>, n = 2^18, Re/Im exponents uniform in
 [-16, 16] (seeded synthetic values, full 1024-bit significands), MPC_RNDNN per
 operation, bit-exact.  One step = one AXPY over the whole array with the
 inputs resident in HBM.  One element = 2 MP ops (one complex multiply, one
 complex add), so value = 2 * n * steps / time.
 
-Also printed in the same JSON line: the roofline of the dominant kernel
-(algorithmic bytes 816 B/element and word products 4096/element over the
-kernel's average launch time, measured with HIP events on the launch stream),
-and the CPU baseline: the reference's own cu_fcomplex<1024> host code
-(oracle/_ref, compiled from the reference header) over all host cores with
-OpenMP, on the same inputs.
+The same JSON line carries, under "configs", every other configuration of the
+metric, each measured in this run with its own roofline and CPU baseline:
+real AXPY (pb 256, n = 2^20, and the sweep over pb 64/128/512/2048), the
+per-op GEMV (pb 512, 4096^2), the per-op and exact GEMM (pb 256, 1024^3,
+exponents in [-4, 4] and [-80, 80]), the 10,000-instance batched mix (fixed
+and tuned mapping), and the rows beside the path (complex GEMM, division and
+square root, run-time precision AXPY).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--extra]
+Rooflines: algorithmic bytes B(pb) = pb/8 + 8 per real value against the
+8 TB/s HBM peak; word products (pb/32)^2 per exact real product against the
+VALU integer multiply-add peak (one v_mad_u64_u32 per lane per clock:
+256 CUs x 64 x 2.4 GHz = 3.93e13/s); int8 operations S^2 2MNK (S = 32
+slices) against the 5.0 POP/s dense INT8 MFMA peak.  Kernel times are HIP
+events on the launch stream.
+
+CPU baselines: system MPFR / MPC in the kernels' operation order, OpenMP over
+the host's cores (oracle/libcpu_mpfr.so, the contender of
+demos/bench3_cpu.cpp), on the same inputs, on a bounded sample stated in each
+record; its results are compared with the GPU's.  The headline also times the
+reference's own cu_fcomplex<1024> header code (oracle/_ref) when it is built.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
 import argparse
 import json
+import math
 import os
+import platform
+import random
 import sys
 import tempfile
 import time
@@ -33,9 +49,18 @@ sys.path.insert(0, ROOT)
 os.environ.setdefault("GPU_MAX_HW_QUEUES", "12")
 
 HBM_PEAK_GBS = 8000.0            # MI355X HBM3E, published
+VALU_WP_PEAK = 256 * 64 * 2.4e9  # v_mad_u64_u32 per lane per clock, 256 CUs
+INT8_PEAK = 5.0e15               # dense INT8 MFMA, published
 PB, N_ELEM = 1024, 1 << 18
-BYTES_PER_ELEM = 3 * 2 * (PB // 8 + 8)        # x, y, z; Re+Im; limbs + sign/exp word = 816
-WORD_PRODUCTS_PER_ELEM = 4 * (PB // 32) ** 2   # four exact 1024x1024-bit products = 4096
+METRIC = "MP ops/s @256/1024-bit: AXPY, GEMM (MFMA exact-dot vs per-op), 0-ULP"
+
+
+def B(pb):
+    return pb // 8 + 8
+
+
+def WP(pb):
+    return (pb // 32) ** 2
 
 
 def main():
@@ -43,11 +68,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--extra", action="store_true", help="also time the other configurations")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--driver", choices=["axpy", "exact", "gemv"], default=None,
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-configs", action="store_true", help="headline only")
+    ap.add_argument("--driver", choices=["axpy", "exact", "gemv", "gemm"], default=None,
                     help="only run a few launches of one kernel (for rocprofv3 --pmc): the headline AXPY "
-                         "plus calibration copies, the exact GEMM at 1024^3, or the GEMV at 4096^2")
+                         "plus calibration copies, the exact GEMM at 1024^3, the GEMV at 4096^2, or the "
+                         "per-op GEMM at 1024^3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -83,10 +109,11 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     value = 2.0 * N_ELEM / (ms_per_step * 1e-3)
 
-    achieved_gbs = BYTES_PER_ELEM * N_ELEM / (kern_ms * 1e-3) / 1e9
-    wp_rate = WORD_PRODUCTS_PER_ELEM * N_ELEM / (kern_ms * 1e-3)
+    bytes_launch = 3 * 2 * B(PB) * N_ELEM
+    achieved_gbs = bytes_launch / (kern_ms * 1e-3) / 1e9
+    wp_rate = 4 * WP(PB) * N_ELEM / (kern_ms * 1e-3)
     rec = {
-        "metric": "MP ops/s @256/1024-bit: AXPY, GEMM (MFMA exact-dot vs per-op), 0-ULP",
+        "metric": METRIC,
         "value": value,
         "unit": "MP ops/s",
         "n_gpus": 1,
@@ -103,29 +130,361 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_axpy_cplx_pipe<1024>", "kernel_ms": kern_ms,
-                     "algorithmic_bytes_per_launch": BYTES_PER_ELEM * N_ELEM,
-                     # rate in 32x32-bit word products of the four exact schoolbook
-                     # products per element (the kernel computes short products)
-                     "equiv_word_products_per_s": wp_rate},
+                     "algorithmic_bytes_per_launch": bytes_launch,
+                     # 32x32-bit word products of the four exact schoolbook products per
+                     # element (the kernel computes short products), against the VALU peak
+                     "equiv_word_products_per_s": wp_rate, "valu_frac": wp_rate / VALU_WP_PEAK},
     }
-
     if not args.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline(a, x, y, z)
-
-    rec["side_metrics"] = side_metrics()
-
-    if args.extra:
-        rec["extra"] = extra_configs(args)
-
+        rec["cpu_baseline"] = cpu_headline(a, x, y, z)
+    if not args.no_configs:
+        rec["configs"] = configs(not args.no_cpu)
     print(json.dumps(rec))
 
 
+# ----------------------------------------------------------------- helpers
+
+def _timed(fn, steps, warm=1):
+    """average milliseconds of fn over `steps` launches, HIP events on the
+    current (launch) stream"""
+    import torch
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(steps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def host_info():
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
+
+
+def _cpu_common():
+    from oracle import oracle as O
+    return {"unit": "MP ops/s", "cores": int(O.cpu().cpu_num_threads()), "kind": "port",
+            "libraries": O.cpu().cpu_versions().decode(), "cpu_model": host_info()}
+
+
+def hbm_roof(bytes_, ms):
+    gbs = bytes_ / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "algorithmic_bytes_per_launch": bytes_}
+
+
+def valu_roof(wp, ms):
+    r = wp / (ms * 1e-3)
+    return {"bound": "valu", "achieved": r, "peak": VALU_WP_PEAK, "unit": "word-products/s",
+            "frac": r / VALU_WP_PEAK, "algorithmic_word_products_per_launch": wp}
+
+
+def cpu_headline(a, x, y, z):
+    """MPC (mpc_mul + mpc_add, MPC_RNDNN) on the whole headline workload, and
+    the reference's own cu_fcomplex<1024> header code when oracle/_ref is
+    built; both checked against the GPU result."""
+    from oracle import oracle as O
+    ap, xp, yp = a.pair(), x.pair(), y.pair()
+    gpu = z.pair()
+    out = {}
+    if O.cpu_available():
+        _, ms1 = O.cpu_axpy(PB, ap, xp, yp, True)
+        reps = max(1, min(200, int(3000.0 / max(ms1, 1e-3))))
+        res, ms = O.cpu_axpy(PB, ap, xp, yp, True, reps=reps)
+        out = dict(_cpu_common())
+        out.update({"value": 2.0 * N_ELEM / (ms * 1e-3),
+                    "sample": f"full workload (n=2^18 complex AXPY at pb=1024, mpc_mul + mpc_add) x {reps} "
+                              f"repetitions, {ms * reps / 1e3:.1f} s of timed loop",
+                    "gpu_vs_cpu_mismatches": O.mismatches(gpu, res, ignore_zero_sign=True)})
+    if O.ref_available():
+        reps, t_total, res = 0, 0.0, None
+        while t_total < 3.0 and reps < 20:
+            t0 = time.perf_counter()
+            res = O.axpy_complex(PB, ap, xp, yp, impl="ref")
+            t_total += time.perf_counter() - t0
+            reps += 1
+        hdr = {"value": 2.0 * N_ELEM * reps / t_total, "unit": "MP ops/s",
+               "cores": int(O.ref().ref_num_threads()), "kind": "reference",
+               "sample": f"full workload x {reps} repetitions, the reference's cu_fcomplex<1024> header code "
+                         f"(OpenMP), {t_total:.1f} s",
+               "gpu_vs_cpu_mismatches": O.mismatches(gpu, res, ignore_zero_sign=False)}
+        if out:
+            out["reference_header"] = hdr
+        else:
+            out = hdr
+    return out or None
+
+
+# ----------------------------------------------------------------- configs
+
+def configs(with_cpu):
+    from oracle import oracle as O
+    with_cpu = with_cpu and O.cpu_available()
+    out = {}
+    out["axpy_real"] = axpy_real_sweep(with_cpu)
+    out["gemv_pb512_4096x4096"] = gemv_config(with_cpu)
+    out.update(gemm_configs(with_cpu))
+    out["batch_10000"] = batch_config(with_cpu)
+    out.update(side_rows())
+    return out
+
+
+def axpy_real_sweep(with_cpu):
+    """Real AXPY out = RNDN(RNDN(a*x) + y), n = 2^20: pb 256 (exponents in
+    [-8, 8], the metric's 256-bit AXPY) and the sweep pb 64/128/512/2048."""
+    import mpc_cuda_amd as M
+    from oracle import oracle as O
+    res = {}
+    for pb in (256, 64, 128, 512, 2048):
+        n = 1 << 20
+        a = M.device_random(pb, 1, 101 + pb, -8, 8)
+        x = M.device_random(pb, n, 102 + pb, -8, 8)
+        y = M.device_random(pb, n, 103 + pb, -8, 8)
+        o = M.empty(pb, n)
+        ms = _timed(lambda: M.axpy(a, x, y, out=o), 20)
+        r = {"ms": ms, "value": 2 * n / (ms * 1e-3), "unit": "MP ops/s",
+             "roofline": hbm_roof(3 * B(pb) * n, ms), "valu": valu_roof(WP(pb) * n, ms)}
+        if with_cpu:
+            ns = n if pb <= 512 else 1 << 17
+            ap, xs, ys = a.pair(), _slice(x, ns), _slice(y, ns)
+            got, cms = O.cpu_axpy(pb, ap, xs, ys, False)
+            c = dict(_cpu_common())
+            c.update({"value": 2 * ns / (cms * 1e-3),
+                      "sample": f"first {ns} elements (mpfr_mul + mpfr_add), {cms:.0f} ms",
+                      "gpu_vs_cpu_mismatches": O.mismatches(_slice(o, ns), got, ignore_zero_sign=True)})
+            r["cpu_baseline"] = c
+        res[f"pb{pb}_n2^20"] = r
+        del a, x, y, o
+    return res
+
+
+def _slice(arr, ns):
+    import numpy as np
+    l, s = arr.limbs[..., :ns], arr.se[..., :ns]
+    return np.ascontiguousarray(l.cpu().numpy().view(np.uint64)), np.ascontiguousarray(s.cpu().numpy())
+
+
+def gemv_config(with_cpu):
+    """GEMV y = A x, pb 512, A 4096 x 4096, exponents in [-4, 4], per-op
+    k-order fold (tools/linalg_fast.cu:37-51)."""
+    import mpc_cuda_amd as M
+    from oracle import oracle as O
+    pb, m, k = 512, 4096, 4096
+    A = M.device_random(pb, m * k, 301, -4, 4)
+    xv = M.device_random(pb, k, 302, -4, 4)
+    yv = M.empty(pb, m)
+    ms = _timed(lambda: M.gemv(A, xv, m, k, out=yv), 3)
+    r = {"ms": ms, "value": 2 * m * k / (ms * 1e-3), "unit": "MP ops/s",
+         "roofline": valu_roof(WP(pb) * m * k, ms), "hbm": hbm_roof((m * k + 2 * m) * B(pb), ms)}
+    if with_cpu:
+        rows = 256
+        Ah = _slice_rows(A, 0, rows * k)
+        got, cms = O.cpu_gemm(pb, rows, 1, k, Ah, xv.pair(), "perop", (0, rows))
+        c = dict(_cpu_common())
+        c.update({"value": 2 * rows * k / (cms * 1e-3),
+                  "sample": f"rows 0..{rows - 1} of 4096 (per-row mpfr_mul + mpfr_add in k order), {cms:.0f} ms",
+                  "gpu_vs_cpu_mismatches": O.mismatches(_slice(yv, rows), got, ignore_zero_sign=True)})
+        r["cpu_baseline"] = c
+    del A
+    return r
+
+
+def _slice_rows(arr, v0, v1):
+    import numpy as np
+    l, s = arr.limbs[..., v0:v1], arr.se[..., v0:v1]
+    return np.ascontiguousarray(l.cpu().numpy().view(np.uint64)), np.ascontiguousarray(s.cpu().numpy())
+
+
+def gemm_configs(with_cpu):
+    """GEMM C = A B, pb 256, 1024^3: per-op RNDN in k order (the reference's
+    matmul) and the exact dot on INT8 MFMA (one rounding per element), with
+    exponents in [-4, 4], and the exact form again with [-80, 80]."""
+    import numpy as np
+    import torch
+    import mpc_cuda_amd as M
+    from oracle import oracle as O
+    pb, d = 256, 1024
+    out = {}
+    A = M.device_random(pb, d * d, 401, -4, 4)
+    Bm = M.device_random(pb, d * d, 402, -4, 4)
+    C = M.empty(pb, d * d)
+    ms_p = _timed(lambda: M.gemm(A, Bm, d, d, d, "perop", out=C), 2)
+    Cp = C.pair()
+    r = {"ms": ms_p, "value": 2 * d ** 3 / (ms_p * 1e-3), "unit": "MP ops/s",
+         "roofline": valu_roof(WP(pb) * d ** 3, ms_p)}
+    Ah, Bh = A.pair(), Bm.pair()
+    if with_cpu:
+        rows = 16
+        got, cms = O.cpu_gemm(pb, d, d, d, Ah, Bh, "perop", (0, rows))
+        c = dict(_cpu_common())
+        c.update({"value": 2 * rows * d * d / (cms * 1e-3),
+                  "sample": f"rows 0..{rows - 1} of 1024 (16384 of 1048576 outputs), {cms:.0f} ms; "
+                            "whole-GEMM CPU time extrapolates x64",
+                  "gpu_vs_cpu_mismatches": O.mismatches(_rows(Cp, 0, rows * d), _rows(got, 0, rows * d),
+                                                        ignore_zero_sign=True)})
+        r["cpu_baseline"] = c
+    out["gemm_pb256_1024^3_perop"] = r
+    for tag, lo, hi in (("", -4, 4), ("_spread80", -80, 80)):
+        if lo != -4:
+            A = M.device_random(pb, d * d, 501, lo, hi)
+            Bm = M.device_random(pb, d * d, 502, lo, hi)
+            Ah, Bh = A.pair(), Bm.pair()
+        ms_e = _timed(lambda: M.gemm(A, Bm, d, d, d, "exact", out=C), 3)
+        ops = 32 * 32 * 2 * d ** 3
+        r = {"ms": ms_e, "value": 2 * d ** 3 / (ms_e * 1e-3), "unit": "MP ops/s",
+             "roofline": {"bound": "mfma", "achieved": ops / (ms_e * 1e-3) / 1e12, "peak": INT8_PEAK / 1e12,
+                          "unit": "TOP/s (int8)", "frac": ops / (ms_e * 1e-3) / INT8_PEAK,
+                          "algorithmic_int8_ops_per_launch": ops,
+                          "note": "S^2 2MNK with S = 32 slices of 8 bits per 256-bit significand"}}
+        if tag == "":
+            r["speedup_vs_perop"] = ms_p / ms_e
+        if with_cpu:
+            rows = 4
+            got, cms = O.cpu_gemm(pb, d, d, d, Ah, Bh, "exact", (0, rows))
+            c = dict(_cpu_common())
+            c.update({"value": 2 * rows * d * d / (cms * 1e-3),
+                      "sample": f"rows 0..{rows - 1} of 1024 (exact products at 512 bits, one mpfr_sum each), "
+                                f"{cms:.0f} ms",
+                      "gpu_vs_cpu_mismatches": O.mismatches(_rows(C.pair(), 0, rows * d), _rows(got, 0, rows * d),
+                                                            ignore_zero_sign=True)})
+            r["cpu_baseline"] = c
+        out[f"gemm_pb256_1024^3_exact{tag}"] = r
+    del A, Bm, C
+    torch.cuda.synchronize()
+    return out
+
+
+def _rows(pair, v0, v1):
+    import numpy as np
+    l, s = pair
+    return np.ascontiguousarray(l[..., v0:v1]), np.ascontiguousarray(s[..., v0:v1])
+
+
+def batch_config(with_cpu, n_tasks=10000, seed=5, check=50):
+    """10,000 instances (mpc_cuda_amd.workloads.batch_mix: AXPY / dot / GEMV at
+    40/40/20 %, half complex, lengths log-uniform in [16, 65536], GEMV rows in
+    [16, 2048], pb in {64, 128, 256, 512, 1024}, exponents in [-8, 8]) in one
+    graph launch; fixed vs tuned mapping; `check` sampled instances verified
+    against the oracle; the CPU baseline times MPFR / MPC on sampled
+    instances and extrapolates by word products."""
+    import torch
+    import mpc_cuda_amd as M
+    from mpc_cuda_amd import batch as Bt
+    from mpc_cuda_amd import workloads as W
+    from oracle import oracle as O
+    insts, tasks = W.batch_mix(n_tasks, seed=seed)
+    torch.cuda.synchronize()
+    mp_ops = sum(2 * x.elements() for x in insts)
+    wp = sum(x.word_products() for x in insts)
+    byt = sum((2 if x.cplx else 1) * B(x.pb) * (3 * x.elements() if x.kind == Bt.AXPY else x.elements() + x.L + x.R)
+              for x in insts)
+    res = {"instances": n_tasks, "mp_ops": mp_ops, "word_products": wp}
+    for label, tuning in (("fixed", False), ("tuned", True)):
+        bt = Bt.Batch(tasks, tuning=tuning,
+                      cache=os.path.join(tempfile.gettempdir(), "mpgfx_batch_tune.json") if tuning else None)
+        ms = _timed(bt.run, 3)
+        res[label] = {"ms": ms, "instances_per_s": n_tasks / (ms * 1e-3), "MP_ops_per_s": mp_ops / (ms * 1e-3),
+                      "roofline": valu_roof(wp, ms), "hbm": hbm_roof(byt, ms),
+                      "scratch_GB": bt.info()["scratch_bytes"] / 1e9}
+        bt.close()
+    rng = random.Random(seed + 1)
+    sample = sorted(rng.sample(range(n_tasks), check))
+    bad = 0
+    for i in sample:
+        x = insts[i]
+        if x.kind == Bt.AXPY:
+            ref = (O.axpy_complex if x.cplx else O.axpy_real)(x.pb, *(v.pair() for v in x.arrays[:3]))
+        else:
+            ref = O.gemm(x.pb, x.R, 1, x.L, x.arrays[0].pair(), x.arrays[1].pair(), "cperop" if x.cplx else "perop")
+        bad += O.mismatches(x.out.pair(), ref, ignore_zero_sign=False)
+    res["sampled_instances_checked"] = check
+    res["sampled_mismatches"] = bad
+    if with_cpu:
+        cms, cwp, cbad = 0.0, 0, 0
+        for i in sample[:25]:
+            x = insts[i]
+            if x.kind == Bt.AXPY:
+                got, ms = O.cpu_axpy(x.pb, *(v.pair() for v in x.arrays[:3]), x.cplx)
+            else:
+                got, ms = O.cpu_gemm(x.pb, x.R, 1, x.L, x.arrays[0].pair(), x.arrays[1].pair(),
+                                     "cperop" if x.cplx else "perop", (0, x.R))
+            cms += ms
+            cwp += x.word_products()
+            cbad += O.mismatches(x.out.pair(), got, ignore_zero_sign=True)
+        c = dict(_cpu_common())
+        c.update({"unit": "instances/s", "value": n_tasks / (cms * 1e-3 * wp / cwp),
+                  "sample": f"25 sampled instances ({cwp:.3g} word products, {cms:.0f} ms of MPFR / MPC), "
+                            f"extrapolated to the batch's {wp:.3g} word products",
+                  "gpu_vs_cpu_mismatches": cbad})
+        res["cpu_baseline"] = c
+    del insts, tasks
+    torch.cuda.synchronize()
+    return res
+
+
+def side_rows():
+    """Rows beside the path: complex GEMM pb 256 1024^3 (per-op and exact),
+    division and square root (pb 256 and 1024, n = 2^18), run-time precision
+    AXPY (1024 and 1000 bits, n = 16384)."""
+    import mpc_cuda_amd as M
+    out = {}
+    pb, d = 256, 1024
+    A = M.device_random(pb, d * d, 511, -4, 4, cplx=True)
+    Bm = M.device_random(pb, d * d, 512, -4, 4, cplx=True)
+    ms_p = _timed(lambda: M.gemm(A, Bm, d, d, d, "perop"), 1)
+    ms_e = _timed(lambda: M.gemm(A, Bm, d, d, d, "exact"), 2)
+    out["cgemm_pb256_1024^3"] = {"perop_ms": ms_p, "exact_ms": ms_e, "exact_speedup": ms_p / ms_e,
+                                 "perop_roofline": valu_roof(4 * WP(pb) * d ** 3, ms_p),
+                                 "exact_MP_ops_per_s": 2 * d ** 3 / (ms_e * 1e-3)}
+    del A, Bm
+    for pb in (256, 1024):
+        n = 1 << 18
+        a = M.device_random(pb, n, 601, -30, 30)
+        b = M.device_random(pb, n, 602, -30, 30)
+        o = M.empty(pb, n)
+        ms_d = _timed(lambda: M.elementwise("div", a, b, out=o), 5)
+        ms_s = _timed(lambda: M.sqrt(a, out=o), 5)
+        out[f"div_sqrt_pb{pb}_n2^18"] = {"div_ms": ms_d, "divisions_per_s": n / (ms_d * 1e-3),
+                                         "sqrt_ms": ms_s, "square_roots_per_s": n / (ms_s * 1e-3)}
+        del a, b, o
+    for p in (1024, 1000):
+        n = 16384
+        a = M.random_real(p, 1, 701, -8, 8).cuda()
+        x = M.random_real(p, n, 702, -8, 8).cuda()
+        y = M.random_real(p, n, 703, -8, 8).cuda()
+        ms_r = _timed(lambda: M.rt_axpy(a, x, y), 10)
+        ca = M.random_complex(p, 1, 704, -8, 8).cuda()
+        cx = M.random_complex(p, n, 705, -8, 8).cuda()
+        cy = M.random_complex(p, n, 706, -8, 8).cuda()
+        ms_c = _timed(lambda: M.rt_axpy(ca, cx, cy), 10)
+        out[f"rt_axpy_p{p}_n16384"] = {"real_ms": ms_r, "complex_ms": ms_c,
+                                       "real_MP_ops_per_s": 2 * n / (ms_r * 1e-3),
+                                       "complex_MP_ops_per_s": 2 * n / (ms_c * 1e-3)}
+    return out
+
+
+# ----------------------------------------------------------------- profiling
+
 def profile_driver(which):
-    """Launches for the rocprofv3 --pmc passes summarised under profiles/:
+    """Launches for the rocprofv3 passes summarised under profiles/:
     "axpy": three headline AXPYs on device-generated inputs, then three 512 MiB
     device copies whose FETCH_SIZE / WRITE_SIZE calibrate the counters;
     "exact": three exact GEMMs pb 256 1024^3 (exponents in [-4, 4]);
-    "gemv": three per-op GEMVs pb 512 4096^2."""
+    "gemv": three per-op GEMVs pb 512 4096^2; "gemm": two per-op GEMMs pb 256
+    1024^3."""
     import torch
     import mpc_cuda_amd as M
     if which == "axpy":
@@ -141,10 +500,17 @@ def profile_driver(which):
     elif which == "exact":
         d = 1024
         A = M.device_random(256, d * d, 401, -4, 4)
-        B = M.device_random(256, d * d, 402, -4, 4)
+        Bm = M.device_random(256, d * d, 402, -4, 4)
         C = M.empty(256, d * d)
         for _ in range(3):
-            M.gemm(A, B, d, d, d, "exact", out=C)
+            M.gemm(A, Bm, d, d, d, "exact", out=C)
+    elif which == "gemm":
+        d = 1024
+        A = M.device_random(256, d * d, 401, -4, 4)
+        Bm = M.device_random(256, d * d, 402, -4, 4)
+        C = M.empty(256, d * d)
+        for _ in range(2):
+            M.gemm(A, Bm, d, d, d, "perop", out=C)
     else:
         m = k = 4096
         A = M.device_random(512, m * k, 301, -4, 4)
@@ -166,216 +532,6 @@ def measured_traffic():
         return int(m.group(1)) if m else None
     except OSError:
         return None
-
-
-def cpu_baseline(a, x, y, z):
-    """The reference's cu_fcomplex<1024> AXPY compiled from its own header
-    (oracle/_ref), OpenMP over all host cores, on the same inputs as the GPU
-    run; its output is also checked against the GPU result."""
-    import numpy as np
-    from oracle import oracle as O
-    ap, xp, yp = a.pair(), x.pair(), y.pair()
-    kind = "reference" if O.ref_available() else "port"
-    impl = "ref" if kind == "reference" else "oracle"
-    cores = O.ref().ref_num_threads() if kind == "reference" else O.lib().or_num_threads()
-    reps, t_total = 0, 0.0
-    out = None
-    while t_total < 8.0 and reps < 50:
-        t0 = time.perf_counter()
-        out = O.axpy_complex(PB, ap, xp, yp, impl=impl)
-        t_total += time.perf_counter() - t0
-        reps += 1
-    rate = 2.0 * N_ELEM * reps / t_total
-    bad = O.mismatches(z.pair(), out, ignore_zero_sign=False)
-    return {"value": rate, "unit": "MP ops/s", "cores": int(cores), "kind": kind,
-            "sample": f"full workload (n=2^18 complex AXPY at pb=1024) x {reps} repetitions, "
-                      f"{t_total:.1f} s wall on {cores} threads",
-            "gpu_vs_cpu_mismatches": int(bad)}
-
-
-def _timed(fn, steps):
-    import torch
-    fn()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(steps):
-        fn()
-    e1.record()
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / steps
-
-
-def side_metrics():
-    """The other halves of the metric, measured in the same run: real AXPY at
-    256 bits (n = 2^20) and the 1024^3 GEMM at 256 bits, per-op vs MFMA
-    exact-dot (exponents in [-4, 4])."""
-    import mpc_cuda_amd as M
-    out = {}
-    pb, n = 256, 1 << 20
-    a = M.device_random(pb, 1, 101, -8, 8)
-    x = M.device_random(pb, n, 102, -8, 8)
-    y = M.device_random(pb, n, 103, -8, 8)
-    o = M.empty(pb, n)
-    ms = _timed(lambda: M.axpy(a, x, y, out=o), 20)
-    out["axpy_real_pb256_n2^20"] = {"ms": ms, "MP_ops_per_s": 2 * n / (ms * 1e-3),
-                                    "GB_per_s": 120 * n / (ms * 1e-3) / 1e9, "hbm_frac": 120 * n / (ms * 1e-3) / 8e12}
-    pb, d = 256, 1024
-    A = M.device_random(pb, d * d, 401, -4, 4)
-    B = M.device_random(pb, d * d, 402, -4, 4)
-    C = M.empty(pb, d * d)
-    ms_p = _timed(lambda: M.gemm(A, B, d, d, d, "perop", out=C), 1)
-    ms_e = _timed(lambda: M.gemm(A, B, d, d, d, "exact", out=C), 2)
-    out["gemm_pb256_1024^3"] = {"perop_ms": ms_p, "exact_ms": ms_e, "exact_speedup": ms_p / ms_e,
-                                "perop_MP_ops_per_s": 2 * d ** 3 / (ms_p * 1e-3),
-                                "exact_MP_ops_per_s": 2 * d ** 3 / (ms_e * 1e-3)}
-    return out
-
-
-def extra_configs(args):
-    """Every other configuration, on GPU-generated inputs: real AXPY over the
-    precision sweep (n = 2^20), GEMV 4096^2 at 512 bits, the exact GEMM with
-    exponent spread [-80, 80], the complex GEMM 1024^3 at 256 bits (per-op and
-    exact), division and square root at 256 and 1024 bits (n = 2^18), the
-    run-time precision AXPY at 1024 and 1000 bits (n = 16384), and the
-    10,000-instance batched workload."""
-    import mpc_cuda_amd as M
-    out = {}
-    for pb in (64, 128, 512, 1024, 2048):
-        n = 1 << 20
-        a = M.device_random(pb, 1, 1, -8, 8)
-        x = M.device_random(pb, n, 2, -8, 8)
-        y = M.device_random(pb, n, 3, -8, 8)
-        o = M.empty(pb, n)
-        ms = _timed(lambda: M.axpy(a, x, y, out=o), 10)
-        out[f"axpy_real_pb{pb}"] = {"ms": ms, "MP_ops_per_s": 2 * n / (ms * 1e-3),
-                                    "GB_per_s": 3 * (pb // 8 + 8) * n / (ms * 1e-3) / 1e9,
-                                    "word_products_per_s": (pb // 32) ** 2 * n / (ms * 1e-3)}
-        del a, x, y, o
-    pb, m, k = 512, 4096, 4096
-    A = M.device_random(pb, m * k, 301, -4, 4)
-    xv = M.device_random(pb, k, 302, -4, 4)
-    ms = _timed(lambda: M.gemv(A, xv, m, k), 2)
-    out["gemv_pb512_4096"] = {"ms": ms, "MP_ops_per_s": 2 * m * k / (ms * 1e-3),
-                              "word_products_per_s": m * k * (pb // 32) ** 2 / (ms * 1e-3)}
-    del A, xv
-    pb, d = 256, 1024
-    A = M.device_random(pb, d * d, 501, -80, 80)
-    B = M.device_random(pb, d * d, 502, -80, 80)
-    ms = _timed(lambda: M.gemm(A, B, d, d, d, "exact"), 2)
-    out["gemm_exact_pb256_1024^3_spread80"] = {"ms": ms, "MP_ops_per_s": 2 * d ** 3 / (ms * 1e-3)}
-    del A, B
-    # complex GEMM pb 256, 1024^3: per-op (MPC_RNDNN per operation) vs exact
-    # per component (one rounding); one complex multiply-add = 2 MP ops
-    A = M.device_random(pb, d * d, 511, -4, 4, cplx=True)
-    B = M.device_random(pb, d * d, 512, -4, 4, cplx=True)
-    ms_p = _timed(lambda: M.gemm(A, B, d, d, d, "perop"), 1)
-    ms_e = _timed(lambda: M.gemm(A, B, d, d, d, "exact"), 2)
-    out["cgemm_pb256_1024^3"] = {"perop_ms": ms_p, "exact_ms": ms_e, "exact_speedup": ms_p / ms_e,
-                                 "exact_MP_ops_per_s": 2 * d ** 3 / (ms_e * 1e-3)}
-    del A, B
-    # division and square root (cu_fdiv / cu_sqrt: Newton at pb + 128 bits),
-    # n = 2^18 values, results per second
-    for pb in (256, 1024):
-        n = 1 << 18
-        a = M.device_random(pb, n, 601, -30, 30)
-        b = M.device_random(pb, n, 602, -30, 30)
-        o = M.empty(pb, n)
-        ms_d = _timed(lambda: M.elementwise("div", a, b, out=o), 5)
-        ms_s = _timed(lambda: M.sqrt(a, out=o), 5)
-        out[f"div_sqrt_pb{pb}_n2^18"] = {"div_ms": ms_d, "divisions_per_s": n / (ms_d * 1e-3),
-                                         "sqrt_ms": ms_s, "square_roots_per_s": n / (ms_s * 1e-3)}
-        del a, b, o
-    # run-time precision AXPY (the reference's MPFR / MPC device demos,
-    # N = 16384): at 1024 bits and at 1000 bits (not a compiled precision);
-    # host-generated inputs
-    for p in (1024, 1000):
-        n = 16384
-        a = M.random_real(p, 1, 701, -8, 8).cuda()
-        x = M.random_real(p, n, 702, -8, 8).cuda()
-        y = M.random_real(p, n, 703, -8, 8).cuda()
-        ms_r = _timed(lambda: M.rt_axpy(a, x, y), 10)
-        ca = M.random_complex(p, 1, 704, -8, 8).cuda()
-        cx = M.random_complex(p, n, 705, -8, 8).cuda()
-        cy = M.random_complex(p, n, 706, -8, 8).cuda()
-        ms_c = _timed(lambda: M.rt_axpy(ca, cx, cy), 10)
-        out[f"rt_axpy_p{p}_n16384"] = {"real_ms": ms_r, "complex_ms": ms_c,
-                                       "real_MP_ops_per_s": 2 * n / (ms_r * 1e-3),
-                                       "complex_MP_ops_per_s": 2 * n / (ms_c * 1e-3)}
-        del a, x, y, ca, cx, cy
-    out["batch_10000"] = batch_config(args)
-    return out
-
-
-def batch_config(args, n_tasks=10000, seed=5, check=50):
-    """10,000 instances: AXPY / dot / GEMV at 40/40/20 %, half complex, lengths
-    log-uniform in [16, 65536], GEMV rows log-uniform in [16, 2048] with the
-    matrix capped at 2^18 values (columns reduced), precision uniform in
-    {64, 128, 256, 512, 1024}, exponents in [-8, 8]; one graph launch.  Fixed
-    mapping vs tuned mapping; `check` sampled instances are verified against
-    the oracle."""
-    import math
-    import random
-    import torch
-    import mpc_cuda_amd as M
-    from mpc_cuda_amd import batch as B
-    rng = random.Random(seed)
-    logu = lambda a, b: int(round(2 ** rng.uniform(math.log2(a), math.log2(b))))
-    tasks, keep, specs = [], [], []
-    mp_ops, wp = 0, 0
-    for t in range(n_tasks):
-        kind = rng.choices([B.AXPY, B.DOT, B.GEMV], [0.4, 0.4, 0.2])[0]
-        pb = rng.choice([64, 128, 256, 512, 1024])
-        cplx = rng.random() < 0.5
-        L = logu(16, 65536)
-        R = 1
-        if kind == B.GEMV:
-            R = logu(16, 2048)
-            L = max(1, min(L, (1 << 18) // R))
-        g = lambda cnt, s: M.device_random(pb, cnt, s, -8, 8, cplx)
-        sd = 7919 * t + 13
-        if kind == B.AXPY:
-            arrs = (g(1, sd), g(L, sd + 1), g(L, sd + 2), M.empty(pb, L, cplx))
-            tasks.append(B.axpy(*arrs))
-            elems = L
-        elif kind == B.DOT:
-            arrs = (g(L, sd), g(L, sd + 1), M.empty(pb, 1, cplx))
-            tasks.append(B.dot(*arrs))
-            elems = L
-        else:
-            arrs = (g(R * L, sd), g(L, sd + 1), M.empty(pb, R, cplx))
-            tasks.append(B.gemv(arrs[0], arrs[1], R, L, arrs[2]))
-            elems = R * L
-        specs.append((kind, pb, cplx, R, L, arrs))
-        mp_ops += 2 * elems
-        wp += elems * (pb // 32) ** 2 * (4 if cplx else 1)
-    torch.cuda.synchronize()
-    res = {"instances": n_tasks, "mp_ops": mp_ops, "word_products": wp}
-    for label, tuning in (("fixed", False), ("tuned", True)):
-        bt = B.Batch(tasks, tuning=tuning,
-                     cache=os.path.join(tempfile.gettempdir(), "mpgfx_batch_tune.json") if tuning else None)
-        ms = _timed(bt.run, 3)
-        res[label] = {"ms": ms, "instances_per_s": n_tasks / (ms * 1e-3), "MP_ops_per_s": mp_ops / (ms * 1e-3),
-                      "word_products_per_s": wp / (ms * 1e-3), "scratch_GB": bt.info()["scratch_bytes"] / 1e9}
-        bt.close()
-    if check:
-        from oracle import oracle as O
-        bad = 0
-        for i in sorted(rng.sample(range(n_tasks), check)):
-            kind, pb, cplx, R, L, arrs = specs[i]
-            if kind == B.AXPY:
-                ref = (O.axpy_complex if cplx else O.axpy_real)(pb, arrs[0].pair(), arrs[1].pair(), arrs[2].pair())
-                got = arrs[3]
-            elif kind == B.DOT:
-                ref = O.gemm(pb, 1, 1, L, arrs[0].pair(), arrs[1].pair(), "cperop" if cplx else "perop")
-                got = arrs[2]
-            else:
-                ref = O.gemm(pb, R, 1, L, arrs[0].pair(), arrs[1].pair(), "cperop" if cplx else "perop")
-                got = arrs[2]
-            bad += O.mismatches(got.pair(), ref, ignore_zero_sign=False)
-        res["sampled_instances_checked"] = check
-        res["sampled_mismatches"] = bad
-    return res
 
 
 if __name__ == "__main__":

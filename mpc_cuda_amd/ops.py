@@ -148,12 +148,30 @@ def gemm(A, B, m, n, k, mode="perop", out=None, workspace=None):
         return out
     md = {"perop": _lib.GEMM_PEROP, "exact": _lib.GEMM_EXACT}[mode]
     ws_ptr, ws_bytes = None, 0
+    if workspace is None and mode == "exact":
+        workspace = _workspace(A.se.device, gemm_workspace_size(A.pb, mode, m, n, k))
     if workspace is not None:
         ws_ptr, ws_bytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
     _lib.check("mpgfx_gemm_real",
                L.mpgfx_gemm_real(A.pb, md, m, n, k, *A.ptrs(), *B.ptrs(), *out.ptrs(),
                                  ws_ptr, ws_bytes, _stream()))
     return out
+
+
+_WS = {}
+
+
+def _workspace(device, nbytes):
+    """a per-device workspace tensor kept between calls and grown on demand
+    (torch's caching allocator), so repeated exact GEMMs do not allocate"""
+    import torch
+    key = str(device)
+    t = _WS.get(key)
+    if t is None or t.numel() < nbytes:
+        _WS.pop(key, None)
+        t = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
+        _WS[key] = t
+    return t
 
 
 def gemm_workspace_size(pb, mode, m, n, k):

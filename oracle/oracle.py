@@ -7,7 +7,9 @@ this module.  It wraps
   (oracle/mpfixed.c),
 * ``oracle/_ref/libref.so`` -- the reference's own header types compiled from
   /root/reference (optional; absent where the reference is not mounted),
-* ``oracle/build/pin_mpfr`` -- the MPFR/MPC semantics driver (optional).
+* ``oracle/build/pin_mpfr`` -- the MPFR/MPC semantics driver (optional),
+* ``oracle/libcpu_mpfr.so`` -- the CPU baseline of bench.py: MPFR/MPC in the
+  kernels' operation order, OpenMP over the host's cores (optional).
 
 Arrays use the structure-of-arrays layout shared with the HIP library: a real
 array of n values at precision pb is ``limbs`` (uint64, shape (N, n)) plus
@@ -85,6 +87,60 @@ def ref():
         L.ref_num_threads.restype = ctypes.c_int
         _REF = L
     return _REF
+
+
+_CPU = None
+
+
+def cpu_available():
+    return os.path.exists(os.path.join(HERE, "libcpu_mpfr.so"))
+
+
+def cpu():
+    """oracle/libcpu_mpfr.so: timed MPFR/MPC loops (the CPU baseline)."""
+    global _CPU
+    if _CPU is None:
+        path = os.path.join(HERE, "libcpu_mpfr.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/libcpu_mpfr.so is not built (make -C oracle cpu)")
+        L = ctypes.CDLL(path)
+        arr2 = [_u64p, _i64p]
+        for f in (L.cpu_axpy_real, L.cpu_axpy_complex):
+            f.argtypes = [ctypes.c_int, ctypes.c_int64] + arr2 * 4
+            f.restype = ctypes.c_double
+        for f in (L.cpu_gemm_perop, L.cpu_cgemm_perop, L.cpu_gemm_exact):
+            f.argtypes = [ctypes.c_int] + [ctypes.c_int64] * 3 + arr2 * 3 + [ctypes.c_int64] * 2
+            f.restype = ctypes.c_double
+        L.cpu_num_threads.restype = ctypes.c_int
+        L.cpu_set_reps.argtypes = [ctypes.c_int]
+        L.cpu_versions.restype = ctypes.c_char_p
+        _CPU = L
+    return _CPU
+
+
+def cpu_axpy(pb, a, x, y, cplx, reps=1):
+    """MPFR (mpfr_mul + mpfr_add) or MPC (mpc_mul + mpc_add) AXPY over the
+    whole arrays: (result, milliseconds of the timed loop, averaged over
+    `reps` runs of it)"""
+    n = x[1].shape[-1]
+    cpu().cpu_set_reps(reps)
+    ol, ose = (empty_complex if cplx else empty_real)(pb, n)
+    f = cpu().cpu_axpy_complex if cplx else cpu().cpu_axpy_real
+    ms = f(pb, n, _p(a[0], _u64p), _p(a[1], _i64p), _p(x[0], _u64p), _p(x[1], _i64p),
+           _p(y[0], _u64p), _p(y[1], _i64p), _p(ol, _u64p), _p(ose, _i64p))
+    return (ol, ose), ms
+
+
+def cpu_gemm(pb, m, n, k, A, B, mode, rows):
+    """MPFR / MPC GEMM on output rows rows=(r0, r1): mode perop | cperop |
+    exact; (result with the other rows zero, milliseconds of the timed loop)"""
+    r0, r1 = rows
+    cpu().cpu_set_reps(1)
+    Cl, Cse = (empty_complex if mode == "cperop" else empty_real)(pb, m * n)
+    f = {"perop": cpu().cpu_gemm_perop, "cperop": cpu().cpu_cgemm_perop, "exact": cpu().cpu_gemm_exact}[mode]
+    ms = f(pb, m, n, k, _p(A[0], _u64p), _p(A[1], _i64p), _p(B[0], _u64p), _p(B[1], _i64p),
+           _p(Cl, _u64p), _p(Cse, _i64p), r0, r1)
+    return (Cl, Cse), ms
 
 
 def pin_available():
