@@ -17,6 +17,7 @@
  */
 #include "common.h"
 #include "dispatch.h"
+#include "coop_dpp.h"
 
 #ifndef MPGFX_PB
 #error "compile with -DMPGFX_PB=<precision in bits>"
@@ -274,6 +275,23 @@ __global__ __launch_bounds__ (TPB) void k_gemm_cplx (i64 m, i64 n, i64 k, const 
 constexpr int COOP_G = coop_lanes (PB);
 constexpr int COOP_R = 64 / COOP_G;
 
+/* the fold of one group of chains: the DPP form for lane groups of at most
+ * 16 (coop_dpp.h), the ds_bpermute form above (coop.h) */
+__device__ __forceinline__ cval fold_chain (const u64 *__restrict__ P, i64 rec0, i64 step, i64 len, i64 max_len)
+{
+  if constexpr (mpgfx::coopd::lanes_for (PB) != 0)
+    {
+      const mpgfx::coopd::cv a = mpgfx::coopd::fold<PB> (mpgfx::coopd::DevLanes{}, P, rec0, step, len, max_len);
+      cval r;
+      r.m = a.m;
+      r.exp = a.exp;
+      r.sign = a.sign;
+      return r;
+    }
+  else
+    return coop_fold<PB> (P, rec0, step, len, max_len);
+}
+
 /* GEMV, per-op, in two phases.  Phase 1: every product P[i,k] = RNDN(A[i,k] *
  * x[k]) in parallel (lane per product), into records of N limbs + the
  * sign/exponent word, ordered for the fold: rows in fold groups of R =
@@ -302,8 +320,79 @@ __global__ __launch_bounds__ (64) void k_gemv_fold (i64 m, i64 k, i64 r0, i64 mc
   const i64 fg = blockIdx.x;
   const int r = (int) (threadIdx.x / COOP_G);
   const i64 i = fg * COOP_R + r;
-  const cval acc = coop_fold<PB> (P, fg * k * COOP_R + r, COOP_R, i < mc ? k : 0, k);
+  const cval acc = fold_chain (P, fg * k * COOP_R + r, COOP_R, i < mc ? k : 0, k);
   if (i < mc) coop_store<PB> (acc, yl, yse, m, r0 + i);
+}
+
+/* GEMV with the products computed inside the fold (lane groups of at most
+ * 16, coop_dpp.h): a wavefront folds R = 64/G rows.  Per block of G steps
+ * every lane computes one product, RNDN(A[i_r, s0+j] x[s0+j]) for its row r
+ * and step s0 + j, and writes it as a record (N limbs, sign/exponent word)
+ * to LDS; the G steps of the block then fold the records of their row, read
+ * from LDS at the start of the block.  The next block's operands are loaded
+ * from HBM while the current block folds.  No scratch memory. */
+constexpr int FG_G = mpgfx::coopd::lanes_for (PB) ? mpgfx::coopd::lanes_for (PB) : 16;
+constexpr int FG_R = 64 / FG_G;
+
+__global__ __launch_bounds__ (64) void k_gemv_fused (i64 m, i64 k, const u64 *__restrict__ Al,
+                                                     const i64 *__restrict__ Ase, const u64 *__restrict__ xl,
+                                                     const i64 *__restrict__ xse, u64 *__restrict__ yl,
+                                                     i64 *__restrict__ yse)
+{
+  if constexpr (mpgfx::coopd::lanes_for (PB) != 0)
+    {
+      namespace cd = mpgfx::coopd;
+      constexpr int G = FG_G, R = FG_R, REC = NL + 1;
+      __shared__ u64 lds[R * G * REC];
+      const cd::DevLanes w;
+      const int lane = (int) threadIdx.x, r = lane / G, j = lane % G;
+      const int jj = j < 1 ? 0 : (j > NL ? NL - 1 : j - 1);
+      const i64 i = (i64) blockIdx.x * R + r, ii = i < m ? i : m - 1;
+      const i64 nA = m * k;
+      cd::cv acc = cd::zero ();
+      /* operands of this lane's first product */
+      i64 kk = j < k ? j : k - 1;
+      cu_freal<PB> a = ld<PB> (Al, Ase, nA, ii * k + kk), xv = ldc<PB> (xl, xse, k, kk);
+      for (i64 s0 = 0; s0 < k; s0 += G)
+        {
+          const cu_freal<PB> p = cu_fp::cu_fmul<PB> (a, xv);
+          u64 *rec = lds + (r * G + j) * REC;
+#pragma unroll
+          for (int l = 0; l < NL; l++) rec[l] = p.m[l];
+          rec[NL] = (u64) (((i64) p.exp << 1) | (p.sign < 0 ? 1 : 0));
+          __syncthreads ();
+          /* this block's records of the row (lane j' holds limb j'-1) */
+          cd::raw q[G];
+#pragma unroll
+          for (int d = 0; d < G; d++)
+            {
+              const u64 *src = lds + (r * G + d) * REC;
+              q[d].m = src[jj];
+              q[d].w = (i64) src[NL];
+            }
+          /* the next block's operands, loaded while this block folds */
+          const i64 sn = s0 + G + j;
+          kk = sn < k ? sn : k - 1;
+          a = ld<PB> (Al, Ase, nA, ii * k + kk);
+          xv = ldc<PB> (xl, xse, k, kk);
+          const i64 steps = k - s0;
+#pragma unroll
+          for (int d = 0; d < G; d++)
+            {
+              const cd::cv nv = cd::add<PB> (w, acc, cd::decode<PB> (w, q[d]));
+              const bool live = d < steps;
+              acc.m = live ? nv.m : acc.m;
+              acc.exp = live ? nv.exp : acc.exp;
+              acc.sign = live ? nv.sign : acc.sign;
+            }
+          __syncthreads ();
+        }
+      cval out;
+      out.m = acc.m;
+      out.exp = acc.exp;
+      out.sign = acc.sign;
+      if (i < m) coop_store<PB> (out, yl, yse, m, i);
+    }
 }
 
 /* AoS (reference struct layout) <-> SoA, one lane per real value */
@@ -413,6 +502,13 @@ int gemv (i64 m, i64 k, const u64 *Al, const i64 *Ase, const u64 *xl, const i64 
       /* empty sums: +0 (the per-op GEMM kernel writes them) */
       hipLaunchKernelGGL (k_gemm_perop, dim3 ((unsigned) (m < 65536 ? m : 65536)), dim3 (TPB), 0, s, m, (i64) 1,
                           k, Al, Ase, xl, xse, yl, yse);
+      return launch_status ("gemv_real");
+    }
+  if constexpr (mpgfx::coopd::lanes_for (PB) != 0)
+    {
+      /* products inside the fold: no scratch */
+      hipLaunchKernelGGL (k_gemv_fused, dim3 ((unsigned) ((m + FG_R - 1) / FG_R)), dim3 (64), 0, s, m, k, Al, Ase, xl,
+                          xse, yl, yse);
       return launch_status ("gemv_real");
     }
   /* product scratch for as many fold groups as the budget holds (one at least) */

@@ -65,6 +65,20 @@ def edge_pairs(pb, seed=1, n_random=64):
     # ties for multiplication: (2^(pb-1)+1)^2 lands near halfway for small pb
     cases.append((val(top + 1, 0), val(top + 1, 0)))
     cases.append((val(top + (1 << (pb // 2)), 0), val(top + (1 << (pb // 2)), 0)))
+    # ties that only the sticky material decides: a +- (half an ulp of a)(1 + 2^-t)
+    # for a of even and odd last bit, the extra bit t places below the half ulp
+    # (inside the guard limb, just below it, far below: the sticky borrow of a
+    # subtraction and the sticky of an addition); and a carry-out whose
+    # dropped bit decides the tie: (2^pb - 1) + 2 (1 + 2^-65) ulps
+    for t in sorted({1, 2, 31, 32, 63, 64, 65, 66, 100, pb - 2, pb - 1}):
+        if 1 <= t <= pb - 1:
+            half = val(top | (1 << (pb - 1 - t)), -pb)
+            for a in (val(top | 2, 0), val(top | 3, 0), val(ones - 1, 0), val(ones, 0)):
+                cases.append((a, half))
+                cases.append((a, (-1, half[1], half[2], half[3])))
+    if pb >= 67:
+        cases.append((val(ones, 0), val(top | (1 << (pb - 66)), 2 - pb)))
+        cases.append((val(ones - 2, 0), val(top | (1 << (pb - 66)), 2 - pb)))
     for a, b in cases:
         A.append(a); B.append(b)
         A.append(b); B.append(a)

@@ -280,3 +280,14 @@ def test_pack_unpack_roundtrip(gpu, pb, cplx):
     y = M.empty(pb, n, cplx)
     assert L.mpgfx_pack(pb, int(cplx), n, aos.data_ptr(), *y.ptrs(), s) == 0
     assert M.equal(x, y) == 0
+
+
+@pytest.mark.parametrize("pb", [32, 64, 96, 128, 192, 256, 320, 384, 512, 1024])
+@pytest.mark.parametrize("mm,kk", [(1, 1), (2, 3), (5, 16), (3, 17), (9, 33), (4, 100)])
+def test_gemv_small_and_ragged(gpu, pb, mm, kk):
+    """GEMV shapes around the fold's blocks: fewer rows than a wavefront's
+    groups, k below, at and just above one block of lane-group steps"""
+    A = M.random_real(pb, mm * kk, 91 + pb + kk, -40, 40)
+    x = M.random_real(pb, kk, 92 + pb + mm, -40, 40)
+    r = M.gemv(A.cuda(), x.cuda(), mm, kk)
+    assert same(r, O.gemm(pb, mm, 1, kk, A.pair(), x.pair(), "perop")) == 0
