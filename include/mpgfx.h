@@ -233,6 +233,22 @@ int mpgfx_rt_axpy_complex (int prec, int64_t n,
                            const uint64_t *y_limbs, const int64_t *y_se,
                            uint64_t *z_limbs, int64_t *z_se, void *stream);
 
+/* Run-time precision per-operation folds (any prec in 1 .. 2048): the
+ * reference's runtime-path matvec / matmul (demos/matvec_mpfr.cu,
+ * demos/matmul_mpfr.cu: mpfr_mul then mpfr_add per k; demos/matvec_mpc.cu,
+ * demos/matmul_mpc.cu: mpc_mul then mpc_add), acc from +0, k ascending, RNDN
+ * (MPC_RNDNN) at prec with MPFR's zero signs.  cplx selects complex MP
+ * arrays.  mpgfx_rt_gemv: y = A x (A m x k); a dot product is m = 1.
+ * mpgfx_rt_gemm: C = A B (A m x k, B k x n).  k = 0 gives +0. */
+int mpgfx_rt_gemv (int prec, int cplx, int64_t m, int64_t k,
+                   const uint64_t *A_limbs, const int64_t *A_se,
+                   const uint64_t *x_limbs, const int64_t *x_se,
+                   uint64_t *y_limbs, int64_t *y_se, void *stream);
+int mpgfx_rt_gemm (int prec, int cplx, int64_t m, int64_t n, int64_t k,
+                   const uint64_t *A_limbs, const int64_t *A_se,
+                   const uint64_t *B_limbs, const int64_t *B_se,
+                   uint64_t *C_limbs, int64_t *C_se, void *stream);
+
 /* n seeded synthetic nonzero values (value i = start + i of the stream keyed by
  * seed: significand limbs random with the top bit set and the low 64N-pb bits
  * clear, random sign, exponent uniform in [emin, emax]) -- the generator of the

@@ -26,6 +26,8 @@ SIGNATURES = {
     "mpgfx_rt_elementwise_real": (_i, [_i, _i, _l] + [_p] * 6 + [_p]),
     "mpgfx_rt_axpy_real": (_i, [_i, _l] + [_p] * 8 + [_p]),
     "mpgfx_rt_axpy_complex": (_i, [_i, _l] + [_p] * 8 + [_p]),
+    "mpgfx_rt_gemv": (_i, [_i, _i, _l, _l] + [_p] * 6 + [_p]),
+    "mpgfx_rt_gemm": (_i, [_i, _i, _l, _l, _l] + [_p] * 6 + [_p]),
     "mpgfx_axpy_real": (_i, [_i, _l] + [_p] * 8 + [_p]),
     "mpgfx_axpy_complex": (_i, [_i, _l] + [_p] * 8 + [_p]),
     "mpgfx_gemv_real": (_i, [_i, _l, _l] + [_p] * 6 + [_p]),

@@ -300,3 +300,47 @@ extern "C" int mpgfx_rt_axpy_complex (int prec, int64_t n, const uint64_t *al, c
   NEED (al && ase && xl && xse && yl && yse && zl && zse);
   return rt_get (prec)->axpy_cplx (prec, n, al, ase, xl, xse, yl, yse, zl, zse, (hipStream_t) stream);
 }
+
+/* zero outputs of an empty sum (k = 0): +0, sign/exponent words only */
+namespace {
+__global__ void k_rt_zero (int64_t n, int nl, uint64_t *l, int64_t *se)
+{
+  const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int q = 0; q < nl; q++) l[(int64_t) q * n + i] = 0;
+  se[i] = (int64_t) ((uint64_t) MPGFX_ZERO_EXP << 1);
+}
+int rt_zero_fill (int prec, int64_t n, int parts, uint64_t *l, int64_t *se, hipStream_t s)
+{
+  const int nl = (prec + 63) / 64;
+  for (int q = 0; q < parts; q++)
+    hipLaunchKernelGGL (k_rt_zero, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, s, n, nl, l + (int64_t) q * nl * n,
+                        se + (int64_t) q * n);
+  return launch_status ("rt_zero");
+}
+} // namespace
+
+extern "C" int mpgfx_rt_gemv (int prec, int cplx, int64_t m, int64_t k, const uint64_t *Al, const int64_t *Ase,
+                              const uint64_t *xl, const int64_t *xse, uint64_t *yl, int64_t *yse, void *stream)
+{
+  if (m < 0 || k < 0 || (cplx != 0 && cplx != 1)) return MPGFX_EINVAL;
+  if (int rc = rt_check (prec, m)) return rc;
+  if (m == 0) return MPGFX_OK;
+  NEED (yl && yse);
+  if (k == 0) return rt_zero_fill (prec, m, cplx ? 2 : 1, yl, yse, (hipStream_t) stream);
+  NEED (Al && Ase && xl && xse);
+  return rt_get (prec)->gemv (cplx, prec, m, k, Al, Ase, xl, xse, yl, yse, (hipStream_t) stream);
+}
+
+extern "C" int mpgfx_rt_gemm (int prec, int cplx, int64_t m, int64_t n, int64_t k, const uint64_t *Al,
+                              const int64_t *Ase, const uint64_t *Bl, const int64_t *Bse, uint64_t *Cl, int64_t *Cse,
+                              void *stream)
+{
+  if (m < 0 || n < 0 || k < 0 || (cplx != 0 && cplx != 1)) return MPGFX_EINVAL;
+  if (int rc = rt_check (prec, m)) return rc;
+  if (m == 0 || n == 0) return MPGFX_OK;
+  NEED (Cl && Cse);
+  if (k == 0) return rt_zero_fill (prec, m * n, cplx ? 2 : 1, Cl, Cse, (hipStream_t) stream);
+  NEED (Al && Ase && Bl && Bse);
+  return rt_get (prec)->gemm (cplx, prec, m, n, k, Al, Ase, Bl, Bse, Cl, Cse, (hipStream_t) stream);
+}

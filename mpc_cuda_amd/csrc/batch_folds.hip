@@ -25,18 +25,12 @@ __device__ __forceinline__ void fold_unit (const bt_class &c, i64 fg, int part)
   const bt_chain ch = c.chains[fg * R + r];
   const i64 max_len = c.chains[fg * R].len;                     /* longest first */
   const i64 len = ch.task >= 0 ? ch.len : 0;
+  const coopd::cv a = coopd::fold<PB> (coopd::DevLanes{}, c.scr + (i64) part * c.scr_n * (NL + 1), c.gbase[fg] + r,
+                                       R, len, max_len);
   cval acc;
-  if constexpr (coopd::lanes_for (PB) != 0)
-    {
-      /* lane groups of at most 16: the DPP fold (coop_dpp.h) */
-      const coopd::cv a = coopd::fold<PB> (coopd::DevLanes{}, c.scr + (i64) part * c.scr_n * (NL + 1),
-                                           c.gbase[fg] + r, R, len, max_len);
-      acc.m = a.m;
-      acc.exp = a.exp;
-      acc.sign = a.sign;
-    }
-  else
-    acc = coop_fold<PB> (c.scr + (i64) part * c.scr_n * (NL + 1), c.gbase[fg] + r, R, len, max_len);
+  acc.m = a.m;
+  acc.exp = a.exp;
+  acc.sign = a.sign;
   if (ch.task >= 0)
     {
       const bt_task &t = c.tasks[ch.task];

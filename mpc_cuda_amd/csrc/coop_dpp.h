@@ -61,7 +61,8 @@ constexpr i64 ZERO_EXP = -0x7fffffffffffffLL;
 
 constexpr int lanes_for (int pb)
 {
-  return ((pb + 63) / 64 + 2 <= 4) ? 4 : ((pb + 63) / 64 + 2 <= 8) ? 8 : ((pb + 63) / 64 + 2 <= 16) ? 16 : 0;
+  return ((pb + 63) / 64 + 2 <= 4) ? 4 : ((pb + 63) / 64 + 2 <= 8) ? 8 : ((pb + 63) / 64 + 2 <= 16) ? 16
+       : ((pb + 63) / 64 + 2 <= 32) ? 32 : ((pb + 63) / 64 + 2 <= 64) ? 64 : 0;
 }
 
 /* one lane's part of a number in the frame */
@@ -90,12 +91,39 @@ inline __host__ __device__ u64 dn64 (const L &w, u64 v)
 {
   return ((u64) w.template row_shr<S> ((u32) (v >> 32)) << 32) | w.template row_shr<S> ((u32) v);
 }
-/* lane OFF of the lane's group of G lanes, G in {4, 8, 16} */
+/* lane j of the group reads lane j + 1 (G <= 16: inside the row; wider
+ * groups span rows: the wavefront-wide shift), resp. lane j - 1; the caller
+ * masks the group's edge */
+template <int G, class L>
+inline __host__ __device__ u64 nxt64 (const L &w, u64 v)
+{
+  if constexpr (G <= 16)
+    return up64<1> (w, v);
+  else
+    return ((u64) w.wave_shl1 ((u32) (v >> 32)) << 32) | w.wave_shl1 ((u32) v);
+}
+template <int G, class L>
+inline __host__ __device__ u64 prv64 (const L &w, u64 v)
+{
+  if constexpr (G <= 16)
+    return dn64<1> (w, v);
+  else
+    return ((u64) w.wave_shr1 ((u32) (v >> 32)) << 32) | w.wave_shr1 ((u32) v);
+}
+
+/* lane OFF of the lane's group of G lanes */
 template <int OFF, int G, class L>
 inline __host__ __device__ u32 gbc32 (const L &w, u32 v)
 {
   static_assert (OFF >= 0 && OFF < G, "lane inside the group");
-  if constexpr (G == 16)
+  if constexpr (G == 64)
+    return w.readlane (v, OFF);
+  else if constexpr (G == 32)
+    {
+      const u32 x0 = w.readlane (v, OFF), x1 = w.readlane (v, 32 + OFF);
+      return (w.lane () & 32) ? x1 : x0;
+    }
+  else if constexpr (G == 16)
     return w.template row_bcast<OFF> (v);
   else if constexpr (G == 8)
     {
@@ -138,7 +166,7 @@ template <int PB, class L>
 inline __host__ __device__ cv add (const L &w, const cv &a, const cv &b)
 {
   constexpr int NL = (PB + 63) / 64, G = lanes_for (PB), SB = 64 * NL - PB;
-  static_assert (G >= 4 && G <= 16 && NL + 2 <= G, "frame does not fit a lane group of at most 16");
+  static_assert (G >= 4 && G <= 64 && NL + 2 <= G, "frame does not fit a lane group");
   const int lane = w.lane (), j = lane & (G - 1), base = lane & ~(G - 1);
   const bool za = a.exp == ZERO_EXP, zb = b.exp == ZERO_EXP;
 
@@ -169,7 +197,15 @@ inline __host__ __device__ cv add (const L &w, const cv &a, const cv &b)
       lost = all_out && (Bs != 0);
       Bs = all_out ? 0ull : Bs;
       const int lsx = all_out ? 0 : ls;
-      if constexpr (G > 1)
+      if constexpr (G >= 32)
+        {
+          /* groups across rows: one ds_bpermute move by lsx lanes */
+          for (int q = 0; q < lsx && q < G; q++) lost = lost || (j == q && Bs != 0);
+          const int src = j + lsx;
+          u64 v = bc64 (w, Bs, base + (src < G ? src : 0));
+          Bs = (src < G) ? v : 0ull;
+        }
+      if constexpr (G > 1 && G <= 16)
         {
           const bool on = lsx & 1;
           lost = lost || (on && j < 1 && Bs != 0);
@@ -177,7 +213,7 @@ inline __host__ __device__ cv add (const L &w, const cv &a, const cv &b)
           v = (j + 1 < G) ? v : 0ull;
           Bs = on ? v : Bs;
         }
-      if constexpr (G > 2)
+      if constexpr (G > 2 && G <= 16)
         {
           const bool on = (lsx >> 1) & 1;
           lost = lost || (on && j < 2 && Bs != 0);
@@ -185,7 +221,7 @@ inline __host__ __device__ cv add (const L &w, const cv &a, const cv &b)
           v = (j + 2 < G) ? v : 0ull;
           Bs = on ? v : Bs;
         }
-      if constexpr (G > 4)
+      if constexpr (G > 4 && G <= 16)
         {
           const bool on = (lsx >> 2) & 1;
           lost = lost || (on && j < 4 && Bs != 0);
@@ -193,7 +229,7 @@ inline __host__ __device__ cv add (const L &w, const cv &a, const cv &b)
           v = (j + 4 < G) ? v : 0ull;
           Bs = on ? v : Bs;
         }
-      if constexpr (G > 8)
+      if constexpr (G > 8 && G <= 16)
         {
           const bool on = (lsx >> 3) & 1;
           lost = lost || (on && j < 8 && Bs != 0);
@@ -202,7 +238,7 @@ inline __host__ __device__ cv add (const L &w, const cv &a, const cv &b)
           Bs = on ? v : Bs;
         }
     }
-  u64 nb = up64<1> (w, Bs);
+  u64 nb = nxt64<G> (w, Bs);
   nb = (j + 1 < G) ? nb : 0ull;
   const u64 T = bs ? ((Bs >> bs) | (nb << (64 - bs))) : Bs;
   lost = lost || (j == 0 && bs != 0 && (Bs << (64 - bs)) != 0);
@@ -256,7 +292,7 @@ inline __host__ __device__ cv add (const L &w, const cv &a, const cv &b)
   else
     {
       const int lz = clz64 (rt);
-      u64 upn = up64<1> (w, res), dnn = dn64<1> (w, res);
+      u64 upn = nxt64<G> (w, res), dnn = prv64<G> (w, res);
       upn = (j + 1 < G) ? upn : 0ull;
       dnn = (j >= 1) ? dnn : 0ull;
       Rn = carry ? ((res >> 1) | (upn << 63)) : (lz ? ((res << lz) | (dnn >> (64 - lz))) : res);
@@ -400,6 +436,11 @@ struct DevLanes
     return (u32) __builtin_amdgcn_update_dpp (0, (int) v, P, 0xf, 0xf, true);
   }
   __device__ u32 bperm (int src, u32 v) const { return (u32) __builtin_amdgcn_ds_bpermute (src << 2, (int) v); }
+  /* lane i reads lane i+1 / i-1 of the wavefront (0 at the ends) */
+  __device__ u32 wave_shl1 (u32 v) const { return (u32) __builtin_amdgcn_update_dpp (0, (int) v, 0x130, 0xf, 0xf, true); }
+  __device__ u32 wave_shr1 (u32 v) const { return (u32) __builtin_amdgcn_update_dpp (0, (int) v, 0x138, 0xf, 0xf, true); }
+  /* the value of lane `src` (a constant) in every lane */
+  __device__ u32 readlane (u32 v, int src) const { return (u32) __builtin_amdgcn_readlane ((int) v, src); }
 };
 #endif
 

@@ -275,63 +275,15 @@ __global__ __launch_bounds__ (TPB) void k_gemm_cplx (i64 m, i64 n, i64 k, const 
 constexpr int COOP_G = coop_lanes (PB);
 constexpr int COOP_R = 64 / COOP_G;
 
-/* the fold of one group of chains: the DPP form for lane groups of at most
- * 16 (coop_dpp.h), the ds_bpermute form above (coop.h) */
-__device__ __forceinline__ cval fold_chain (const u64 *__restrict__ P, i64 rec0, i64 step, i64 len, i64 max_len)
-{
-  if constexpr (mpgfx::coopd::lanes_for (PB) != 0)
-    {
-      const mpgfx::coopd::cv a = mpgfx::coopd::fold<PB> (mpgfx::coopd::DevLanes{}, P, rec0, step, len, max_len);
-      cval r;
-      r.m = a.m;
-      r.exp = a.exp;
-      r.sign = a.sign;
-      return r;
-    }
-  else
-    return coop_fold<PB> (P, rec0, step, len, max_len);
-}
-
-/* GEMV, per-op, in two phases.  Phase 1: every product P[i,k] = RNDN(A[i,k] *
- * x[k]) in parallel (lane per product), into records of N limbs + the
- * sign/exponent word, ordered for the fold: rows in fold groups of R =
- * 64/COOP_G, record (i/R)*K*R + k*R + i%R, so that a fold step of a group is R
- * consecutive records.  Phase 2: a wavefront per fold group, COOP_G lanes per
- * row, acc = RNDN(acc + P[i,k]) for k ascending (coop_fold). */
-__global__ __launch_bounds__ (TPB) void k_gemv_products (i64 m, i64 k, i64 r0, i64 mc, const u64 *__restrict__ Al,
-                                                         const i64 *__restrict__ Ase, const u64 *__restrict__ xl,
-                                                         const i64 *__restrict__ xse, u64 *__restrict__ P)
-{
-  /* rows r0 .. r0+mc-1 */
-  const i64 nA = m * k, kr = k * COOP_R, nrec = (mc + COOP_R - 1) / COOP_R * kr;
-  const i64 stride = (i64) gridDim.x * blockDim.x;
-  for (i64 e = (i64) blockIdx.x * blockDim.x + threadIdx.x; e < nrec; e += stride)
-    {
-      const i64 fg = e / kr, rem = e - fg * kr, kk = rem / COOP_R, i = fg * COOP_R + (rem - kk * COOP_R);
-      if (i < mc)
-        st_rec<PB> (cu_fp::cu_fmul<PB> (ld<PB> (Al, Ase, nA, (r0 + i) * k + kk), ldc<PB> (xl, xse, k, kk)),
-                P + e * (NL + 1));
-    }
-}
-
-__global__ __launch_bounds__ (64) void k_gemv_fold (i64 m, i64 k, i64 r0, i64 mc, const u64 *__restrict__ P,
-                                                    u64 *__restrict__ yl, i64 *__restrict__ yse)
-{
-  const i64 fg = blockIdx.x;
-  const int r = (int) (threadIdx.x / COOP_G);
-  const i64 i = fg * COOP_R + r;
-  const cval acc = fold_chain (P, fg * k * COOP_R + r, COOP_R, i < mc ? k : 0, k);
-  if (i < mc) coop_store<PB> (acc, yl, yse, m, r0 + i);
-}
-
-/* GEMV with the products computed inside the fold (lane groups of at most
- * 16, coop_dpp.h): a wavefront folds R = 64/G rows.  Per block of G steps
+/* GEMV y_i = fold_k RNDN(acc + RNDN(A[i,k] x[k])), acc from +0, k ascending
+ * (tools/linalg_fast.cu:37-51), with the products computed inside the
+ * cooperative fold (coop_dpp.h): a wavefront folds R = 64/G rows.  Per block of G steps
  * every lane computes one product, RNDN(A[i_r, s0+j] x[s0+j]) for its row r
  * and step s0 + j, and writes it as a record (N limbs, sign/exponent word)
  * to LDS; the G steps of the block then fold the records of their row, read
  * from LDS at the start of the block.  The next block's operands are loaded
  * from HBM while the current block folds.  No scratch memory. */
-constexpr int FG_G = mpgfx::coopd::lanes_for (PB) ? mpgfx::coopd::lanes_for (PB) : 16;
+constexpr int FG_G = mpgfx::coopd::lanes_for (PB);
 constexpr int FG_R = 64 / FG_G;
 
 __global__ __launch_bounds__ (64) void k_gemv_fused (i64 m, i64 k, const u64 *__restrict__ Al,
@@ -339,7 +291,7 @@ __global__ __launch_bounds__ (64) void k_gemv_fused (i64 m, i64 k, const u64 *__
                                                      const i64 *__restrict__ xse, u64 *__restrict__ yl,
                                                      i64 *__restrict__ yse)
 {
-  if constexpr (mpgfx::coopd::lanes_for (PB) != 0)
+  static_assert (FG_G >= 4, "precision beyond the cooperative frame");
     {
       namespace cd = mpgfx::coopd;
       constexpr int G = FG_G, R = FG_R, REC = NL + 1;
@@ -486,14 +438,6 @@ int gemm_cplx (i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl
   hipLaunchKernelGGL (k_gemm_cplx, dim3 (g), dim3 (TPB), 0, s, m, n, k, Al, Ase, Bl, Bse, Cl, Cse);
   return launch_status ("gemm_complex");
 }
-/* product scratch budget of one GEMV call: 2 GiB, or MPGFX_GEMV_SCRATCH_BYTES */
-i64 gemv_scratch_budget ()
-{
-  const char *e = getenv ("MPGFX_GEMV_SCRATCH_BYTES");
-  const long long v = e ? atoll (e) : 0;
-  return v > 0 ? (i64) v : ((i64) 2 << 30);
-}
-
 int gemv (i64 m, i64 k, const u64 *Al, const i64 *Ase, const u64 *xl, const i64 *xse, u64 *yl, i64 *yse,
           hipStream_t s)
 {
@@ -504,33 +448,10 @@ int gemv (i64 m, i64 k, const u64 *Al, const i64 *Ase, const u64 *xl, const i64 
                           k, Al, Ase, xl, xse, yl, yse);
       return launch_status ("gemv_real");
     }
-  if constexpr (mpgfx::coopd::lanes_for (PB) != 0)
-    {
-      /* products inside the fold: no scratch */
-      hipLaunchKernelGGL (k_gemv_fused, dim3 ((unsigned) ((m + FG_R - 1) / FG_R)), dim3 (64), 0, s, m, k, Al, Ase, xl,
-                          xse, yl, yse);
-      return launch_status ("gemv_real");
-    }
-  /* product scratch for as many fold groups as the budget holds (one at least) */
-  const i64 per_group = COOP_R * k * (NL + 1) * 8;
-  i64 groups = gemv_scratch_budget () / per_group;
-  if (groups < 1) groups = 1;
-  const i64 mc_max = (groups * COOP_R < m) ? groups * COOP_R : m;
-  const i64 cap = (mc_max + COOP_R - 1) / COOP_R * COOP_R * k;
-  u64 *P = nullptr;
-  hipError_t e = hipMallocAsync ((void **) &P, (size_t) cap * (NL + 1) * 8, s);
-  if (e != hipSuccess) { set_error ("gemv_real scratch", e); return MPGFX_ENOMEM; }
-  for (i64 r0 = 0; r0 < m; r0 += mc_max)
-    {
-      const i64 mc = (m - r0 < mc_max) ? m - r0 : mc_max;
-      const i64 nfg = (mc + COOP_R - 1) / COOP_R;
-      hipLaunchKernelGGL (k_gemv_products, dim3 (grid_for (nfg * COOP_R * k, TPB, 32)), dim3 (TPB), 0, s, m, k, r0,
-                          mc, Al, Ase, xl, xse, P);
-      hipLaunchKernelGGL (k_gemv_fold, dim3 ((unsigned) nfg), dim3 (64), 0, s, m, k, r0, mc, P, yl, yse);
-    }
-  int rc = launch_status ("gemv_real");
-  (void) hipFreeAsync (P, s);
-  return rc;
+  /* products inside the fold: no scratch */
+  hipLaunchKernelGGL (k_gemv_fused, dim3 ((unsigned) ((m + FG_R - 1) / FG_R)), dim3 (64), 0, s, m, k, Al, Ase, xl, xse,
+                      yl, yse);
+  return launch_status ("gemv_real");
 }
 int pack (int cplx, i64 n, const void *aos, u64 *l, i64 *se, hipStream_t s)
 {

@@ -163,6 +163,10 @@ struct rt_ops
                     i64 *, hipStream_t);
   int (*axpy_cplx) (int p, i64 n, const u64 *, const i64 *, const u64 *, const i64 *, const u64 *, const i64 *, u64 *,
                     i64 *, hipStream_t);
+  int (*gemv) (int cplx, int p, i64 m, i64 k, const u64 *, const i64 *, const u64 *, const i64 *, u64 *, i64 *,
+               hipStream_t);
+  int (*gemm) (int cplx, int p, i64 m, i64 n, i64 k, const u64 *, const i64 *, const u64 *, const i64 *, u64 *, i64 *,
+               hipStream_t);
 };
 const rt_ops *rt_ops_cw_2 ();
 const rt_ops *rt_ops_cw_4 ();

@@ -101,6 +101,34 @@ def rt_axpy(a, x, y, out=None):
     return out
 
 
+def rt_gemv(A, x, m, k, out=None):
+    """Run-time precision GEMV y = A x (any A.pb in 1..2048), real or complex:
+    the per-op fold with mpfr_mul + mpfr_add (mpc_mul + mpc_add) at A.pb."""
+    _dev(A), _dev(x)
+    if A.pb != x.pb or A.n != m * k or x.n != k or A.cplx != x.cplx:
+        raise ValueError("shape mismatch")
+    out = _out(out, A.pb, m, A.cplx, A)
+    _lib.check("mpgfx_rt_gemv", _lib.lib().mpgfx_rt_gemv(A.pb, int(A.cplx), m, k, *A.ptrs(), *x.ptrs(), *out.ptrs(),
+                                                         _stream()))
+    return out
+
+
+def rt_dot(x, y, out=None):
+    """Run-time precision dot product (the per-op fold), a GEMV of one row."""
+    return rt_gemv(x, y, 1, x.n, out=out)
+
+
+def rt_gemm(A, B, m, n, k, out=None):
+    """Run-time precision GEMM C = A B, per-op fold per element at A.pb."""
+    _dev(A), _dev(B)
+    if A.pb != B.pb or A.n != m * k or B.n != k * n or A.cplx != B.cplx:
+        raise ValueError("shape mismatch")
+    out = _out(out, A.pb, m * n, A.cplx, A)
+    _lib.check("mpgfx_rt_gemm", _lib.lib().mpgfx_rt_gemm(A.pb, int(A.cplx), m, n, k, *A.ptrs(), *B.ptrs(),
+                                                         *out.ptrs(), _stream()))
+    return out
+
+
 def axpy(a, x, y, out=None):
     """out = RNDN(RNDN(a*x) + y) with a scalar (an MPArray of one value)."""
     _dev(a), _dev(x), _dev(y)

@@ -4,8 +4,9 @@
  * broadcast, quad permutation, bpermute) an exchange through shared slots
  * between two barriers, with the lane mappings of gfx950 (row_shl:n -- lane i
  * reads lane i+n of its 16-lane row, 0 past the row; row_shr:n -- lane i-n;
- * row_newbcast:n -- lane n of the row; quad_perm; ds_bpermute -- the lane
- * given by the address).  Test harness only. */
+ * row_newbcast:n -- lane n of the row; quad_perm; wave_shl1 / wave_shr1 --
+ * lane i+1 / i-1 of the wavefront; readlane; ds_bpermute -- the lane given by
+ * the address).  Test harness only. */
 #include <atomic>
 #include <barrier>
 #include <cstdint>
@@ -61,6 +62,9 @@ struct EmuLanes
   template <int S> u32 row_bcast (u32 v, int line = __builtin_LINE ()) const { return xchg (v, (l & ~15) + S, true, line); }
   template <int P> u32 quad_perm (u32 v, int line = __builtin_LINE ()) const { return xchg (v, (l & ~3) + ((P >> (2 * (l & 3))) & 3), true, line); }
   u32 bperm (int src, u32 v, int line = __builtin_LINE ()) const { return xchg (v, src & 63, true, line); }
+  u32 wave_shl1 (u32 v, int line = __builtin_LINE ()) const { return xchg (v, l + 1, l + 1 < 64, line); }
+  u32 wave_shr1 (u32 v, int line = __builtin_LINE ()) const { return xchg (v, l - 1, l >= 1, line); }
+  u32 readlane (u32 v, int src, int line = __builtin_LINE ()) const { return xchg (v, src & 63, true, line); }
 };
 
 /* chains laid out as the GEMV's fold groups: R = 64 / G chains per wavefront,
@@ -109,7 +113,7 @@ extern "C" int emul_fold (int pb, int64_t nchains, const int64_t *lens, const in
                           const int64_t *wave_max, const u64 *recs, u64 *ol, int64_t *ose)
 {
 #define C(P) if (pb == P) { run<P> (nchains, lens, rec0, step, wave_max, recs, ol, ose); return 0; }
-  C (32) C (64) C (96) C (128) C (160) C (192) C (256) C (288) C (320) C (384) C (512)
+  C (32) C (64) C (96) C (128) C (160) C (192) C (256) C (288) C (320) C (384) C (512) C (1024) C (1056) C (2048)
 #undef C
   return -1;
 }

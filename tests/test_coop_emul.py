@@ -105,7 +105,7 @@ def oracle_fold(pb, chains):
     return np.ascontiguousarray(al), np.ascontiguousarray(ase)
 
 
-@pytest.mark.parametrize("pb", [32, 64, 96, 128, 192, 256, 320, 512])
+@pytest.mark.parametrize("pb", [32, 64, 96, 128, 192, 256, 320, 512, 1024, 1056, 2048])
 def test_dpp_fold_emulated_vs_oracle(emul, pb):
     G = emul.emul_lanes(pb)
     R = 64 // G

@@ -228,14 +228,13 @@ def test_gemv_wide_exponents(gpu, pb):
     assert same(r, O.gemm(pb, mm, 1, kk, A.pair(), x.pair(), "perop")) == 0
 
 
-def test_gemv_row_chunks(gpu, monkeypatch):
-    """GEMV with a product-scratch budget of one 64-row group: rows are
-    processed in chunks (200 rows = 3 full groups + 8), same results."""
+def test_gemv_partial_wavefront(gpu):
+    """GEMV of 200 rows at pb 128 (16 rows per wavefront: 12 full wavefronts
+    and one of 8 rows) and k = 33 (two full blocks of 16 steps and one step)."""
     pb, mm, kk = 128, 200, 33
     A = M.random_real(pb, mm * kk, 51, -4, 4)
     x = M.random_real(pb, kk, 52, -4, 4)
     ref = O.gemm(pb, mm, 1, kk, A.pair(), x.pair(), "perop")
-    monkeypatch.setenv("MPGFX_GEMV_SCRATCH_BYTES", "1")
     r = M.gemv(A.cuda(), x.cuda(), mm, kk)
     torch.cuda.synchronize()
     assert same(r, ref) == 0
