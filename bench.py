@@ -129,10 +129,11 @@ def main():
                    "pb": PB, "n": N_ELEM, "mp_ops_per_element": 2},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "k_axpy_cplx_pipe<1024>", "kernel_ms": kern_ms,
+                     "kernel": "k_axpy_cplx_mfma<1024>", "kernel_ms": kern_ms,
                      "algorithmic_bytes_per_launch": bytes_launch,
                      # 32x32-bit word products of the four exact schoolbook products per
-                     # element (the kernel computes short products), against the VALU peak
+                     # element, against the VALU peak (an equivalent rate: the kernel forms
+                     # the products' top digits as int8 digit products on MFMA)
                      "equiv_word_products_per_s": wp_rate, "valu_frac": wp_rate / VALU_WP_PEAK},
     }
     if not args.no_cpu:

@@ -18,6 +18,7 @@
 #include "common.h"
 #include "dispatch.h"
 #include "coop_dpp.h"
+#include "axpy_mfma.h"
 
 #ifndef MPGFX_PB
 #error "compile with -DMPGFX_PB=<precision in bits>"
@@ -209,6 +210,265 @@ __global__ __launch_bounds__ (TPB) void k_axpy_cplx_pipe (i64 nchunks, i64 n, co
           st<PB> (z, zl + p * o, zse + p * n, n, c * 32 + e);
           if (cn >= nchunks) break;
           c = cn;
+        }
+    }
+}
+
+/* The complex AXPY with its products on INT8 MFMA (axpy_mfma.h): whole
+ * 32-element chunks, one chunk per wavefront at a time, AXM_WAVES wavefronts
+ * per workgroup sharing the A tiles and the words of a_r and a_i in LDS, two
+ * wavefronts per SIMD.  Per wavefront, LDS holds a word-value buffer through
+ * which a product's words pass in two groups of output tiles (it first holds
+ * the chunk's recoded x digits, from which the B tiles are read). */
+constexpr bool AXM = PB == 1024;
+constexpr int AXM_WAVES = 8;
+constexpr int AXM_TPB = 64 * AXM_WAVES;
+namespace axm {
+typedef axmf::cfg<AXM ? PB : 1024> C;
+constexpr int TG = (C::NT + 1) / 2;                    /* output tiles per pass through LDS */
+constexpr int NG = (C::NT + TG - 1) / TG;
+constexpr int PBS = 64 + 2;                            /* word-value row: 64 lanes, padded */
+constexpr int PBUF = 4 * TG * PBS * 8;
+constexpr int DGS = C::KW + 4;                         /* digit row stride (words), 16-byte rows */
+constexpr int WAVE = PBUF;
+constexpr int ATAB = 2 * C::NF * 64 * 16;              /* A tiles [q][tile][lane], 16 bytes a lane */
+constexpr int TPS = 2 * C::TPW * 4;                    /* padded digit strings of a_r, a_i */
+constexpr int AWS = 2 * C::KW * 4;                     /* words of a_r, a_i */
+constexpr int LDSB = AXM ? ATAB + AXM_WAVES * WAVE + TPS + AWS : 16;
+static_assert (!AXM || (64 * DGS * 4 <= PBUF && C::KW == cu_freal<PB>::KW && cu_freal<PB>::SB == 0), "layout");
+static_assert (LDSB <= 160 * 1024, "LDS");
+}
+
+/* LDS written by some lanes of a wavefront and read by others: a wavefront's
+ * LDS instructions execute in order, so only the compiler must keep them in
+ * program order */
+__device__ __forceinline__ void wave_lds_sync ()
+{
+  __builtin_amdgcn_fence (__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier ();
+  __builtin_amdgcn_fence (__ATOMIC_ACQUIRE, "wavefront");
+}
+
+typedef int v4i __attribute__ ((ext_vector_type (4)));
+
+/* MFMA tiles [TA, TB) of pass q for every x part and column tile; lane l
+ * writes the word values of its rows for the lane that carries the product */
+template <int Q, int TA, int TB>
+__device__ __forceinline__ void axm_tiles (const v4i *atab, const v4i (&bf)[2][2][axm::C::KB], i64 *pw, int L)
+{
+  using cu_fp::core::sfor;
+  typedef axm::C C;
+  const int col = L & 15, g = L >> 4;
+  /* the A tiles of this group */
+  v4i af[(TB - TA) * C::KB];
+  sfor<TA, TB> ([&] (auto Tc) {
+    constexpr int T = decltype (Tc)::value;
+    sfor<0, C::KB> ([&] (auto Bc) {
+      constexpr int b = decltype (Bc)::value;
+      if constexpr (C::live (T, b)) af[(T - TA) * C::KB + b] = atab[(Q * C::NF + C::fidx (T, b)) * 64 + L];
+    });
+  });
+#pragma unroll
+  for (int pp = 0; pp < 2; pp++)
+#pragma unroll
+    for (int nt = 0; nt < 2; nt++)
+      {
+        v4i acc[TB - TA];
+#pragma unroll
+        for (int T = 0; T < TB - TA; T++) acc[T] = (v4i) {0, 0, 0, 0};
+        /* K block outer: the two products into one accumulator are far apart */
+        sfor<0, C::KB> ([&] (auto Bc) {
+          constexpr int b = decltype (Bc)::value;
+          sfor<TA, TB> ([&] (auto Tc) {
+            constexpr int T = decltype (Tc)::value;
+            if constexpr (C::live (T, b))
+              acc[T - TA] = __builtin_amdgcn_mfma_i32_16x16x64_i8 (af[(T - TA) * C::KB + b], bf[pp][nt][b],
+                                                                   acc[T - TA], 0, 0, 0);
+          });
+        });
+        const int tl = 2 * (nt * 16 + col) + (pp ^ Q);
+#pragma unroll
+        for (int T = 0; T < TB - TA; T++)
+          pw[(4 * T + g) * axm::PBS + tl] = axmf::assemble (acc[T][0], acc[T][1], acc[T][2], acc[T][3]);
+      }
+}
+
+/* RNDN(a_r x_r - a_i x_i) (p = 0) or RNDN(a_r x_i + a_i x_r) (p = 1) of
+ * element i from the exact products (rare: called, so that its registers do
+ * not count against the loop's) */
+__device__ __attribute__ ((noinline)) cu_freal<PB> axm_exact (const u64 *al, const i64 *ase, const u64 *xl,
+                                                              const i64 *xse, i64 n, i64 i, int p)
+{
+  const cu_freal<PB> ar = ldc<PB> (al, ase, 1, 0), ai = ldc<PB> (al + NL, ase + 1, 1, 0);
+  const cu_freal<PB> xr = ldc<PB> (xl, xse, n, i), xi = ldc<PB> (xl + (i64) NL * n, xse + n, n, i);
+  return cu_fp::core::fmm_exact<PB> (ar, p ? xi : xr, ai, p ? xr : xi, p == 0);
+}
+
+template <int P>
+__global__ __launch_bounds__ (AXM_TPB) void k_axpy_cplx_mfma (i64 nchunks, i64 n, const u64 *__restrict__ al,
+                                                              const i64 *__restrict__ ase, const u64 *__restrict__ xl,
+                                                              const i64 *__restrict__ xse, const u64 *__restrict__ yl,
+                                                              const i64 *__restrict__ yse, u64 *__restrict__ zl,
+                                                              i64 *__restrict__ zse)
+{
+  static_assert (P == PB, "one instantiation, at the library's precision");
+  __shared__ __attribute__ ((aligned (16))) unsigned char lds[axm::LDSB];
+  if constexpr (P == 1024)
+    {
+      using namespace axm;
+      typedef axmf::cfg<P> C;
+      using cu_fp::core::sfor;
+      constexpr int KW = C::KW, NF = C::NF, NT = C::NT, KB = C::KB, H = C::H, TPW = C::TPW;
+      const int tid = threadIdx.x, L = tid & 63, w = __builtin_amdgcn_readfirstlane (tid >> 6);
+      const i64 o = (i64) NL * n;
+      v4i *const atab = (v4i *) lds;
+      u32 *const tpw = (u32 *) (lds + ATAB + AXM_WAVES * WAVE);
+      u32 *const awl = (u32 *) (lds + ATAB + AXM_WAVES * WAVE + TPS);
+
+      /* a_r, a_i: words, padded signed-digit strings, then their A tiles */
+      if (tid < 2 * KW) awl[tid] = ((const u32 *) al)[tid];
+      if (tid < 2 * TPW)
+        {
+          const int q = tid >= TPW ? 1 : 0;
+          u32 aq[KW], d[KW];
+          ldc<P> (al + q * NL, ase + q, 1, 0).words (aq);
+          axmf::recode<KW> (d, aq);
+          tpw[tid] = axmf::tp_dword<P> (tid - q * TPW, d);
+        }
+      __syncthreads ();
+      sfor<0, NT> ([&] (auto Tc) {
+        constexpr int T = decltype (Tc)::value;
+        sfor<0, KB> ([&] (auto Bc) {
+          constexpr int b = decltype (Bc)::value;
+          if constexpr (C::live (T, b))
+            {
+#pragma unroll
+              for (int q = 0; q < 2; q++)
+                {
+                  const int f = q * NF + C::fidx (T, b);
+                  if (f % AXM_WAVES == w)
+                    {
+                      const int k0 = axmf::frag_k0<P> (T, b, L);
+                      v4i fr;
+#pragma unroll
+                      for (int i = 0; i < 4; i++) fr[i] = (int) axmf::frag_dword (tpw + q * TPW, k0, i);
+                      atab[f * 64 + L] = fr;
+                    }
+                }
+            }
+        });
+      });
+      __syncthreads ();
+
+      const int e = L >> 1, p = L & 1, col = L & 15, g = L >> 4;
+      /* a's sign/exponent words */
+      const i64 sea[2] = {ase[0], ase[1]};
+      i64 *const pw = (i64 *) (lds + ATAB + w * WAVE);
+      u32 *const dg = (u32 *) pw;
+      const i64 wstride = (i64) gridDim.x * AXM_WAVES;
+      for (i64 c = (i64) blockIdx.x * AXM_WAVES + w; c < nchunks; c += wstride)
+        {
+          const i64 i = c * 32 + e;
+          /* this lane's x part (one per-lane pointer stepped by a plane, kept
+           * opaque so that the compiler holds no address per plane), both
+           * parts' sign/exponent words */
+          const i64 se0 = xse[i], se1 = xse[n + i];
+          {
+            u32 xw[KW], dgw[KW];
+            const u64 *xp = xl + p * o + i;
+#pragma unroll
+            for (int k = 0; k < NL; k++)
+              {
+                const u64 v = __builtin_nontemporal_load (xp);
+                xp += n;
+                asm volatile ("" : "+v"(xp));
+                xw[2 * k] = (u32) v;
+                xw[2 * k + 1] = (u32) (v >> 32);
+              }
+            axmf::recode<KW> (dgw, xw);
+            u32 *row = dg + (p * 32 + e) * DGS;
+#pragma unroll
+            for (int j = 0; j < KW / 4; j++)
+              *(v4i *) (row + 4 * j) = (v4i) {(int) dgw[4 * j], (int) dgw[4 * j + 1], (int) dgw[4 * j + 2],
+                                              (int) dgw[4 * j + 3]};
+          }
+          wave_lds_sync ();
+          /* B tiles: x digits of both parts, both column tiles, every K block */
+          v4i bf[2][2][KB];
+#pragma unroll
+          for (int pp = 0; pp < 2; pp++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+              for (int b = 0; b < KB; b++)
+                bf[pp][nt][b] = *(const v4i *) (dg + (pp * 32 + nt * 16 + col) * DGS + 16 * b + 4 * g);
+          wave_lds_sync ();
+          axmf::term<P> tm[2];
+          cu_freal<P> yv;
+          sfor<0, 2> ([&] (auto Qc) {
+            constexpr int q = decltype (Qc)::value;
+            if constexpr (q == 1)
+              {
+                const u64 *yp = yl + p * o + i;
+#pragma unroll
+                for (int k = 0; k < NL; k++)
+                  {
+                    yv.m[k] = __builtin_nontemporal_load (yp);
+                    yp += n;
+                    asm volatile ("" : "+v"(yp));
+                  }
+                const i64 yw = __builtin_nontemporal_load (yse + p * n + i);
+                yv.sign = (yw & 1) ? -1 : 1;
+                yv.exp = (long) (yw >> 1);
+              }
+            axmf::carry<P> cs;
+            cs.c = 0;
+            auto aw = [&] (int j) { return awl[q * KW + j]; };
+            sfor<0, NG> ([&] (auto Gc) {
+              constexpr int TA = decltype (Gc)::value * TG, TB = (TA + TG < NT) ? TA + TG : NT;
+              axm_tiles<q, TA, TB> (atab, bf, pw, L);
+              wave_lds_sync ();
+              axmf::sweep_range<PB, 4 * TA, 4 * TB> (cs, [&] (int ww) { return pw[ww * PBS + L]; }, aw);
+              wave_lds_sync ();
+            });
+            /* this lane's product: a_q times x part p ^ q of element e */
+            const i64 sew = (p ^ q) ? se1 : se0;
+            const i64 aqe = sea[q] >> 1;
+            axmf::term<P> &t = tm[q];
+            const bool zero = aqe == (i64) CU_FREAL_ZERO_EXP || (sew >> 1) == (i64) CU_FREAL_ZERO_EXP;
+            t.exp = aqe + (sew >> 1);
+            axmf::sweep_finish<P> (t, cs);
+            t.zero = zero;
+            t.sign = zero ? 1 : (((sea[q] ^ sew) & 1) ? -1 : 1);
+            t.exp = zero ? 0 : t.exp;
+            t.odd = t.odd && !zero;
+#pragma unroll
+            for (int j = 0; j < H; j++) t.M[j] = zero ? 0u : t.M[j];
+          });
+          /* RNDN(p0 - p1) (real part, p = 0) or RNDN(p0 + p1), one rounding */
+          tm[1].sign = (p == 0) ? -tm[1].sign : tm[1].sign;
+          const cu_fp::core::rres<KW> r = cu_fp::core::add_round<KW, H, axmf::ERRB> (
+            tm[0].sign, tm[0].exp, tm[0].zero, tm[0].M, tm[1].sign, tm[1].exp, tm[1].zero, tm[1].M);
+          const bool amb = r.amb || tm[0].odd || tm[1].odd;
+          cu_freal<P> prod;
+          prod.set_kept (r.k);
+          prod.sign = r.sign;
+          prod.exp = r.exp;
+          if (r.zero) prod.set_zero ();
+          /* the exact products for the lanes the bound leaves open (rare) */
+          if (__builtin_amdgcn_ballot_w64 (amb) != 0 && amb) prod = axm_exact (al, ase, xl, xse, n, i, p);
+          const cu_freal<P> z = cu_fp::cu_fadd<P> (prod, yv);
+          {
+            u64 *zp = zl + p * o + i;
+#pragma unroll
+            for (int k = 0; k < NL; k++)
+              {
+                __builtin_nontemporal_store (z.m[k], zp);
+                zp += n;
+                asm volatile ("" : "+v"(zp));
+              }
+            __builtin_nontemporal_store ((i64) ((u64) (i64) z.exp << 1) | (z.sign < 0 ? 1 : 0), zse + p * n + i);
+          }
         }
     }
 }
@@ -410,7 +670,16 @@ int axpy_cplx (i64 n, const u64 *al, const i64 *ase, const u64 *xl, const i64 *x
                u64 *ol, i64 *ose, hipStream_t s)
 {
   i64 i0 = 0;
-  if (AXC_PIPE && n >= 32)
+  if (AXM && n >= 32)
+    {
+      /* one workgroup per CU (its LDS) */
+      const i64 nchunks = n / 32, wg = (nchunks + AXM_WAVES - 1) / AXM_WAVES;
+      const i64 grid = wg < 256 ? wg : 256;
+      hipLaunchKernelGGL ((k_axpy_cplx_mfma<PB>), dim3 ((unsigned) grid), dim3 (AXM_TPB), 0, s, nchunks, n, al, ase, xl, xse,
+                          yl, yse, ol, ose);
+      i0 = nchunks * 32;
+    }
+  else if (AXC_PIPE && n >= 32)
     {
       const i64 nchunks = n / 32, wg = (nchunks + AXC_WAVES - 1) / AXC_WAVES;
       hipLaunchKernelGGL (k_axpy_cplx_pipe, dim3 ((unsigned) (wg < 256 ? wg : 256)), dim3 (TPB), 0, s, nchunks, n, al,
