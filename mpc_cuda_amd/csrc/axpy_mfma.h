@@ -257,6 +257,24 @@ CUFP_INL void sweep (term<PB> &t, VF &&V, const u32 (&aw)[cfg<PB>::KW])
  * the normalising shift (ERR = 1), in the form of core::fmm_short's bound */
 constexpr int ERRB = 1 + 3;
 
+/* RNDN of one normalised term to KW words (the real product a*x): K gets the
+ * kept words, the return value is the carry out (K is then 2^(PB-1) and the
+ * exponent grows by one).  The term is within 2 units of its last word of the
+ * exact product (either side), so the rounding is decided unless the 64 guard
+ * bits below the kept words lie within that of the rounding midpoint (or the
+ * window's top two bits were clear): then `amb` is set and the exact product
+ * must give the result. */
+template <int PB>
+CUFP_INL u32 round_term (u32 (&K)[cfg<PB>::KW], const term<PB> &t, bool &amb)
+{
+  constexpr int KW = cfg<PB>::KW, H = cfg<PB>::H;
+  static_assert (H == KW + 2, "two guard words");
+  const u64 g = ((u64) t.M[1] << 32) | t.M[0];
+  const u64 d = g - (1ull << 63);                 /* wraps: |g - 2^63| <= 4 <=> d + 4 <= 8 */
+  amb = (d + 4u <= 8u) || t.odd;
+  return cu_fp::core::round_top<KW, H> (K, t.M, 0u, 0u);
+}
+
 } // namespace axmf
 } // namespace mpgfx
 

@@ -83,11 +83,11 @@ __global__ __launch_bounds__ (TPB) void k_ew_cplx (i64 n, const u64 *__restrict_
 __global__ __launch_bounds__ (TPB) void k_axpy_real (i64 n, const u64 *__restrict__ al, const i64 *__restrict__ ase,
                                                      const u64 *__restrict__ xl, const i64 *__restrict__ xse,
                                                      const u64 *__restrict__ yl, const i64 *__restrict__ yse,
-                                                     u64 *__restrict__ ol, i64 *__restrict__ ose)
+                                                     u64 *__restrict__ ol, i64 *__restrict__ ose, i64 i0)
 {
   const cu_freal<PB> a = ldc<PB> (al, ase, 1, 0);
   i64 stride = (i64) gridDim.x * blockDim.x;
-  for (i64 i = (i64) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+  for (i64 i = i0 + (i64) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     {
       cu_freal<PB> t = cu_fp::cu_fmul<PB> (a, ld<PB> (xl, xse, n, i));
       st<PB> (cu_fp::cu_fadd<PB> (t, ld<PB> (yl, yse, n, i)), ol, ose, n, i);
@@ -473,6 +473,218 @@ __global__ __launch_bounds__ (AXM_TPB) void k_axpy_cplx_mfma (i64 nchunks, i64 n
     }
 }
 
+/* The real AXPY out = RNDN(RNDN(a x) + y) with the product on INT8 MFMA
+ * (axpy_mfma.h), pb 1024 and 2048: whole 64-element chunks, one lane per
+ * element.  Per chunk: lane e loads x[e], recodes it and writes the digits to
+ * LDS; the B tiles of the four 16-element column tiles are read into
+ * registers; the output tiles are formed in groups of AXR_TG, each lane
+ * writing the word values of its rows for the lane of that element, which
+ * carries them (adding a's words at offset KW); the window is rounded once
+ * (round_term), lanes whose rounding it leaves open take the exact product;
+ * then y is added.  Workgroup: AXR_WAVES wavefronts sharing a's A tiles. */
+constexpr bool AXR = PB == 1024 || PB == 2048;
+constexpr int AXR_WAVES = PB == 2048 ? 4 : 8;
+constexpr int AXR_TPB = 64 * AXR_WAVES;
+namespace axr {
+typedef axmf::cfg<AXR ? PB : 1024> C;
+constexpr int TG = (C::NT + 1) / 2;                    /* output tiles per pass through LDS */
+constexpr int NG = (C::NT + TG - 1) / TG;
+constexpr int PBS = 64 + 2;                            /* word-value row: 64 lanes, padded */
+constexpr int DGS = C::KW + 4;                         /* digit row stride (words), 16-byte rows */
+constexpr int PBUF = (4 * TG * PBS * 8 > 64 * DGS * 4) ? 4 * TG * PBS * 8 : 64 * DGS * 4;
+constexpr int ATAB = C::NF * 64 * 16;                  /* A tiles [tile][lane], 16 bytes a lane */
+constexpr int TPS = C::TPW * 4;                        /* padded digit string of a */
+constexpr int AWS = C::KW * 4;                         /* words of a */
+constexpr int LDSB = AXR ? ATAB + AXR_WAVES * PBUF + TPS + AWS : 16;
+static_assert (!AXR || (C::KW == cu_freal<PB>::KW && cu_freal<PB>::SB == 0), "layout");
+static_assert (LDSB <= 160 * 1024, "LDS");
+}
+
+/* MFMA tiles [TA, TB) of the real product for the four column tiles */
+template <int TA, int TB>
+__device__ __forceinline__ void axr_tiles (const v4i *atab, const v4i (&bf)[4][axr::C::KB], i64 *pw, int L)
+{
+  using cu_fp::core::sfor;
+  typedef axr::C C;
+  const int col = L & 15, g = L >> 4;
+  v4i af[(TB - TA) * C::KB];
+  sfor<TA, TB> ([&] (auto Tc) {
+    constexpr int T = decltype (Tc)::value;
+    sfor<0, C::KB> ([&] (auto Bc) {
+      constexpr int b = decltype (Bc)::value;
+      if constexpr (C::live (T, b)) af[(T - TA) * C::KB + b] = atab[C::fidx (T, b) * 64 + L];
+    });
+  });
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++)
+    {
+      v4i acc[TB - TA];
+#pragma unroll
+      for (int T = 0; T < TB - TA; T++) acc[T] = (v4i) {0, 0, 0, 0};
+      sfor<0, C::KB> ([&] (auto Bc) {
+        constexpr int b = decltype (Bc)::value;
+        sfor<TA, TB> ([&] (auto Tc) {
+          constexpr int T = decltype (Tc)::value;
+          if constexpr (C::live (T, b))
+            acc[T - TA] = __builtin_amdgcn_mfma_i32_16x16x64_i8 (af[(T - TA) * C::KB + b], bf[nt][b], acc[T - TA], 0,
+                                                                 0, 0);
+        });
+      });
+#pragma unroll
+      for (int T = 0; T < TB - TA; T++)
+        pw[(4 * T + g) * axr::PBS + nt * 16 + col] = axmf::assemble (acc[T][0], acc[T][1], acc[T][2], acc[T][3]);
+    }
+}
+
+/* RNDN(a x[i]) from the exact product (rare: called, so that its registers do
+ * not count against the loop's) */
+__device__ __attribute__ ((noinline)) cu_freal<PB> axr_exact (const u64 *al, const i64 *ase, const u64 *xl,
+                                                              const i64 *xse, i64 n, i64 i)
+{
+  return cu_fp::core::fmul_exact<PB> (ldc<PB> (al, ase, 1, 0), ldc<PB> (xl, xse, n, i));
+}
+
+template <int P>
+__global__ __launch_bounds__ (AXR_TPB) void k_axpy_real_mfma (i64 nchunks, i64 n, const u64 *__restrict__ al,
+                                                              const i64 *__restrict__ ase, const u64 *__restrict__ xl,
+                                                              const i64 *__restrict__ xse, const u64 *__restrict__ yl,
+                                                              const i64 *__restrict__ yse, u64 *__restrict__ zl,
+                                                              i64 *__restrict__ zse)
+{
+  static_assert (P == PB, "one instantiation, at the library's precision");
+  __shared__ __attribute__ ((aligned (16))) unsigned char lds[axr::LDSB];
+  if constexpr (P == 1024 || P == 2048)
+    {
+      using namespace axr;
+      using cu_fp::core::sfor;
+      typedef axmf::cfg<P> C;
+      constexpr int KW = C::KW, NT = C::NT, KB = C::KB, H = C::H, TPW = C::TPW;
+      const int tid = threadIdx.x, L = tid & 63, w = __builtin_amdgcn_readfirstlane (tid >> 6);
+      v4i *const atab = (v4i *) lds;
+      u32 *const tpw = (u32 *) (lds + ATAB + AXR_WAVES * PBUF);
+      u32 *const awl = (u32 *) (lds + ATAB + AXR_WAVES * PBUF + TPS);
+
+      /* a: its words, its padded signed-digit string, then its A tiles */
+      if (tid < KW) awl[tid] = ((const u32 *) al)[tid];
+      if (tid < TPW)
+        {
+          u32 aw[KW], d[KW];
+          ldc<P> (al, ase, 1, 0).words (aw);
+          axmf::recode<KW> (d, aw);
+          tpw[tid] = axmf::tp_dword<P> (tid, d);
+        }
+      __syncthreads ();
+      sfor<0, NT> ([&] (auto Tc) {
+        constexpr int T = decltype (Tc)::value;
+        sfor<0, KB> ([&] (auto Bc) {
+          constexpr int b = decltype (Bc)::value;
+          if constexpr (C::live (T, b))
+            {
+              const int f = C::fidx (T, b);
+              if (f % AXR_WAVES == w)
+                {
+                  const int k0 = axmf::frag_k0<P> (T, b, L);
+                  v4i fr;
+#pragma unroll
+                  for (int i = 0; i < 4; i++) fr[i] = (int) axmf::frag_dword (tpw, k0, i);
+                  atab[f * 64 + L] = fr;
+                }
+            }
+        });
+      });
+      __syncthreads ();
+
+      const int col = L & 15, g = L >> 4;
+      const i64 sea = ase[0], aqe = sea >> 1;
+      i64 *const pw = (i64 *) (lds + ATAB + w * PBUF);
+      u32 *const dg = (u32 *) pw;
+      const i64 wstride = (i64) gridDim.x * AXR_WAVES;
+      for (i64 c = (i64) blockIdx.x * AXR_WAVES + w; c < nchunks; c += wstride)
+        {
+          const i64 i = c * 64 + L;
+          const i64 sex = xse[i];
+          {
+            u32 xw[KW], dgw[KW];
+            const u64 *xp = xl + i;
+#pragma unroll
+            for (int k = 0; k < NL; k++)
+              {
+                const u64 v = __builtin_nontemporal_load (xp);
+                xp += n;
+                asm volatile ("" : "+v"(xp));
+                xw[2 * k] = (u32) v;
+                xw[2 * k + 1] = (u32) (v >> 32);
+              }
+            axmf::recode<KW> (dgw, xw);
+            u32 *row = dg + L * DGS;
+#pragma unroll
+            for (int j = 0; j < KW / 4; j++)
+              *(v4i *) (row + 4 * j) = (v4i) {(int) dgw[4 * j], (int) dgw[4 * j + 1], (int) dgw[4 * j + 2],
+                                              (int) dgw[4 * j + 3]};
+          }
+          wave_lds_sync ();
+          /* B tiles: digit words 16b + 4g .. +3 of element 16 nt + col */
+          v4i bf[4][KB];
+#pragma unroll
+          for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+            for (int b = 0; b < KB; b++) bf[nt][b] = *(const v4i *) (dg + (nt * 16 + col) * DGS + 16 * b + 4 * g);
+          wave_lds_sync ();
+          axmf::carry<P> cs;
+          cs.c = 0;
+          auto aw = [&] (int j) { return awl[j]; };
+          sfor<0, NG> ([&] (auto Gc) {
+            constexpr int TA = decltype (Gc)::value * TG, TB = (TA + TG < NT) ? TA + TG : NT;
+            axr_tiles<TA, TB> (atab, bf, pw, L);
+            wave_lds_sync ();
+            axmf::sweep_range<P, 4 * TA, 4 * TB> (cs, [&] (int ww) { return pw[ww * PBS + L]; }, aw);
+            wave_lds_sync ();
+          });
+          cu_freal<P> yv;
+          {
+            const u64 *yp = yl + i;
+#pragma unroll
+            for (int k = 0; k < NL; k++)
+              {
+                yv.m[k] = __builtin_nontemporal_load (yp);
+                yp += n;
+                asm volatile ("" : "+v"(yp));
+              }
+            const i64 yw = __builtin_nontemporal_load (yse + i);
+            yv.sign = (yw & 1) ? -1 : 1;
+            yv.exp = (long) (yw >> 1);
+          }
+          axmf::term<P> t;
+          t.exp = aqe + (sex >> 1);
+          axmf::sweep_finish<P> (t, cs);
+          const bool zero = aqe == (i64) CU_FREAL_ZERO_EXP || (sex >> 1) == (i64) CU_FREAL_ZERO_EXP;
+          u32 K[KW];
+          bool amb;
+          const u32 co = axmf::round_term<P> (K, t, amb);
+          amb = amb && !zero;
+          cu_freal<P> prod;
+          prod.set_kept (K);
+          prod.sign = ((sea ^ sex) & 1) ? -1 : 1;
+          prod.exp = t.exp + (i64) co;
+          if (zero) prod.set_zero ();
+          /* the exact product for the lanes the window leaves open (rare) */
+          if (__builtin_amdgcn_ballot_w64 (amb) != 0 && amb) prod = axr_exact (al, ase, xl, xse, n, i);
+          const cu_freal<P> z = cu_fp::cu_fadd<P> (prod, yv);
+          {
+            u64 *zp = zl + i;
+#pragma unroll
+            for (int k = 0; k < NL; k++)
+              {
+                __builtin_nontemporal_store (z.m[k], zp);
+                zp += n;
+                asm volatile ("" : "+v"(zp));
+              }
+            __builtin_nontemporal_store ((i64) ((u64) (i64) z.exp << 1) | (z.sign < 0 ? 1 : 0), zse + i);
+          }
+        }
+    }
+}
+
 /* C[i,j] = fold_k RNDN(acc + RNDN(A[i,k]*B[k,j])): one lane per C element; the
  * lanes of a workgroup share the row i, so A[i,k] is one broadcast load and
  * B[k, j..j+255] one coalesced row segment per step. */
@@ -663,7 +875,18 @@ int ew_cplx (int op, i64 n, const u64 *al, const i64 *ase, const u64 *bl, const 
 int axpy_real (i64 n, const u64 *al, const i64 *ase, const u64 *xl, const i64 *xse, const u64 *yl, const i64 *yse,
                u64 *ol, i64 *ose, hipStream_t s)
 {
-  hipLaunchKernelGGL (k_axpy_real, dim3 (grid_for (n, TPB, 32)), dim3 (TPB), 0, s, n, al, ase, xl, xse, yl, yse, ol, ose);
+  i64 i0 = 0;
+  if (AXR && n >= 64)
+    {
+      /* one workgroup per CU (its LDS) */
+      const i64 nchunks = n / 64, wg = (nchunks + AXR_WAVES - 1) / AXR_WAVES;
+      hipLaunchKernelGGL ((k_axpy_real_mfma<PB>), dim3 ((unsigned) (wg < 256 ? wg : 256)), dim3 (AXR_TPB), 0, s, nchunks,
+                          n, al, ase, xl, xse, yl, yse, ol, ose);
+      i0 = nchunks * 64;
+    }
+  if (i0 < n)
+    hipLaunchKernelGGL (k_axpy_real, dim3 (grid_for (n - i0, TPB, 32)), dim3 (TPB), 0, s, n, al, ase, xl, xse, yl, yse,
+                        ol, ose, i0);
   return launch_status ("axpy_real");
 }
 int axpy_cplx (i64 n, const u64 *al, const i64 *ase, const u64 *xl, const i64 *xse, const u64 *yl, const i64 *yse,

@@ -189,6 +189,105 @@ long run (int64_t n, const uint64_t *al, const int64_t *ase, const uint64_t *xl,
   return nodd;
 }
 
+/* the real AXPY out = RNDN(RNDN(a x) + y): 64-element chunks, one lane per
+ * element, four column tiles, the output tiles in groups of (NT + 1) / 2 */
+template <int PB>
+long run_real (int64_t n, const uint64_t *al, const int64_t *ase, const uint64_t *xl, const int64_t *xse,
+               const uint64_t *yl, const int64_t *yse, uint64_t *zl, int64_t *zse)
+{
+  typedef cfg<PB> C;
+  constexpr int KW = C::KW, NF = C::NF, NT = C::NT, KB = C::KB, TPW = C::TPW, DGS = KW + 4;
+  constexpr int TG = (NT + 1) / 2;
+  const cu_freal<PB> a = ldv<PB> (al, ase, 1, 0);
+  uint32_t aw[KW];
+  a.words (aw);
+  std::vector<uint32_t> tpw (TPW);
+  {
+    uint32_t d[KW];
+    recode<KW> (d, aw);
+    for (int i = 0; i < TPW; i++) tpw[i] = tp_dword<PB> (i, d);
+  }
+  std::vector<v4> atab (NF * 64);
+  for (int T = 0; T < NT; T++)
+    for (int b = 0; b < KB; b++)
+      if (C::live (T, b))
+        for (int L = 0; L < 64; L++)
+          {
+            const int k0 = frag_k0<PB> (T, b, L);
+            v4 fr;
+            for (int i = 0; i < 4; i++) fr.x[i] = (int32_t) frag_dword (tpw.data (), k0, i);
+            atab[C::fidx (T, b) * 64 + L] = fr;
+          }
+  long nodd = 0;
+  const int64_t nchunks = n / 64;
+  std::vector<uint32_t> dg (64 * DGS);
+  std::vector<int64_t> pw (C::NWW * 64);
+  static v4 bf[4][8][64], acc[40][64];
+  for (int64_t c = 0; c < nchunks; c++)
+    {
+      for (int L = 0; L < 64; L++)
+        {
+          const cu_freal<PB> xv = ldv<PB> (xl, xse, n, c * 64 + L);
+          uint32_t xw[KW], d[KW];
+          xv.words (xw);
+          recode<KW> (d, xw);
+          for (int j = 0; j < KW; j++) dg[L * DGS + j] = d[j];
+        }
+      for (int nt = 0; nt < 4; nt++)
+        for (int b = 0; b < KB; b++)
+          for (int L = 0; L < 64; L++)
+            for (int i = 0; i < 4; i++)
+              bf[nt][b][L].x[i] = (int32_t) dg[(nt * 16 + (L & 15)) * DGS + 16 * b + 4 * (L >> 4) + i];
+      for (int nt = 0; nt < 4; nt++)
+        {
+          for (int T = 0; T < NT; T++)
+            for (int L = 0; L < 64; L++) acc[T][L] = v4{{0, 0, 0, 0}};
+          for (int T = 0; T < NT; T++)
+            for (int b = 0; b < KB; b++)
+              if (C::live (T, b))
+                {
+                  v4 A[64];
+                  for (int L = 0; L < 64; L++) A[L] = atab[C::fidx (T, b) * 64 + L];
+                  mfma (acc[T], A, bf[nt][b]);
+                }
+          for (int L = 0; L < 64; L++)
+            for (int T = 0; T < NT; T++)
+              pw[(4 * T + (L >> 4)) * 64 + nt * 16 + (L & 15)] =
+                assemble (acc[T][L].x[0], acc[T][L].x[1], acc[T][L].x[2], acc[T][L].x[3]);
+        }
+      for (int L = 0; L < 64; L++)
+        {
+          const int64_t i = c * 64 + L;
+          const int64_t sex = xse[i];
+          carry<PB> cs;
+          cs.c = 0;
+          sweep_range<PB, 0, 4 * TG> (cs, [&] (int ww) { return pw[ww * 64 + L]; }, [&] (int j) { return aw[j]; });
+          sweep_range<PB, 4 * TG, 4 * NT> (cs, [&] (int ww) { return pw[(4 * TG + ww) * 64 + L]; },
+                                          [&] (int j) { return aw[j]; });
+          term<PB> t;
+          t.exp = (int64_t) a.exp + (sex >> 1);
+          sweep_finish<PB> (t, cs);
+          const bool zero = a.is_zero () || (sex >> 1) == (int64_t) CU_FREAL_ZERO_EXP;
+          uint32_t K[KW];
+          bool amb;
+          const uint32_t co = round_term<PB> (K, t, amb);
+          amb = amb && !zero;
+          cu_freal<PB> prod;
+          prod.set_kept (K);
+          prod.sign = ((ase[0] ^ sex) & 1) ? -1 : 1;
+          prod.exp = t.exp + (int64_t) co;
+          if (zero) prod.set_zero ();
+          if (amb)
+            {
+              nodd++;
+              prod = cu_fp::core::fmul_exact<PB> (a, ldv<PB> (xl, xse, n, i));
+            }
+          stv<PB> (cu_fp::cu_fadd<PB> (prod, ldv<PB> (yl, yse, n, i)), zl, zse, n, i);
+        }
+    }
+  return nodd;
+}
+
 } // namespace
 
 extern "C" {
@@ -200,6 +299,14 @@ long axmf_emul_axpy (int pb, int64_t n, const uint64_t *al, const int64_t *ase, 
   if (pb == 512) return run<512> (n, al, ase, xl, xse, yl, yse, zl, zse);
   if (pb == 1024) return run<1024> (n, al, ase, xl, xse, yl, yse, zl, zse);
   if (pb == 2048) return run<2048> (n, al, ase, xl, xse, yl, yse, zl, zse);
+  return -1;
+}
+/* real: out = a x + y over the whole 64-element chunks of n */
+long axmf_emul_axpy_real (int pb, int64_t n, const uint64_t *al, const int64_t *ase, const uint64_t *xl,
+                          const int64_t *xse, const uint64_t *yl, const int64_t *yse, uint64_t *zl, int64_t *zse)
+{
+  if (pb == 1024) return run_real<1024> (n, al, ase, xl, xse, yl, yse, zl, zse);
+  if (pb == 2048) return run_real<2048> (n, al, ase, xl, xse, yl, yse, zl, zse);
   return -1;
 }
 }

@@ -7,7 +7,11 @@ including the sign field of zeros.  Operands: random values with exponent
 spreads up to +-200, zero parts of a, x and y, and the products the short
 window cannot decide -- a real part on a rounding midpoint that only the low
 product bits move off, exact cancellation (+0) and cancellation of all but a
-few bits -- which must take the exact products (the emulator counts them)."""
+few bits -- which must take the exact products (the emulator counts them).
+The real AXPY with its product on MFMA (k_axpy_real_mfma, pb 1024 and 2048)
+likewise against the oracle's real AXPY: random values, products just above a
+rounding midpoint (which must take the exact product), exact products, zero
+operands, all-ones significands."""
 import ctypes
 import os
 
@@ -145,3 +149,67 @@ def test_extreme_significands_vs_oracle(emul, pb):
         got, _ = run_emul(emul, pb, a.pair(), x.pair(), y.pair())
         want = O.axpy_complex(pb, a.pair(), x.pair(), y.pair())
         assert O.mismatches(got, want, ignore_zero_sign=False) == 0
+
+
+# ---------------------------------------------------------------- real AXPY
+
+@pytest.fixture(scope="module")
+def emul_real(emul):
+    emul.axmf_emul_axpy_real.restype = ctypes.c_long
+    emul.axmf_emul_axpy_real.argtypes = [ctypes.c_int, ctypes.c_int64] + [u64p, i64p] * 4
+    return emul
+
+
+def run_real(L, pb, a, x, y):
+    n = x[1].shape[0]
+    zl = np.zeros_like(x[0])
+    zs = np.zeros_like(x[1])
+    nodd = L.axmf_emul_axpy_real(pb, n, _p(a[0], u64p), _p(a[1], i64p), _p(x[0], u64p), _p(x[1], i64p),
+                                 _p(y[0], u64p), _p(y[1], i64p), _p(zl, u64p), _p(zs, i64p))
+    assert nodd >= 0
+    return (zl, zs), nodd
+
+
+@pytest.mark.parametrize("pb", [1024, 2048])
+@pytest.mark.parametrize("spread", [8, 300])
+def test_real_random_vs_oracle(emul_real, pb, spread):
+    n = 64 * (6 if pb < 2048 else 3)
+    a = M.random_real(pb, 1, 21 + pb + spread, -spread, spread)
+    x = M.random_real(pb, n, 22 + pb + spread, -spread, spread)
+    y = M.random_real(pb, n, 23 + pb + spread, -spread, spread)
+    got, nodd = run_real(emul_real, pb, a.pair(), x.pair(), y.pair())
+    assert O.mismatches(got, O.axpy_real(pb, a.pair(), x.pair(), y.pair()), ignore_zero_sign=False) == 0
+    assert nodd == 0
+
+
+@pytest.mark.parametrize("pb", [1024, 2048])
+def test_real_midpoints_zeros_extremes_vs_oracle(emul_real, pb):
+    """products just above a rounding midpoint ((2^(pb-1) + 2^(pb/2) + 1)
+    (2^(pb-1) + 2^(pb/2-2) + 1)), exact products of powers of two and of
+    all-ones significands, zero x and y, and products that cancel y"""
+    top, ones = 1 << (pb - 1), (1 << pb) - 1
+    v = lambda m, e, s=1: (s, e, m << (64 * nlimbs(pb) - pb), False)
+    n = 128
+    rnd = M.random_real(pb, n, 60 + pb, -30, 30)
+    X = unpack(rnd)
+    Y = list(reversed(X))
+    a_m = top + (1 << (pb // 2)) + 1
+    for k in range(0, n, 6):
+        X[k] = v(top + (1 << (pb // 2 - 2)) + 1, 3)             # a x near a midpoint
+    for k in range(1, n, 9):
+        X[k] = v(top, -2)                                        # a x exact (a times a power of two)
+    for k in range(2, n, 11):
+        X[k] = ZERO
+    for k in range(3, n, 7):
+        Y[k] = ZERO
+    for k in range(4, n, 13):
+        X[k] = v(ones, 1, -1)
+    a = pack([v(a_m, 2)], pb)
+    x, y = pack(X, pb), pack(Y, pb)
+    got, nodd = run_real(emul_real, pb, a.pair(), x.pair(), y.pair())
+    assert O.mismatches(got, O.axpy_real(pb, a.pair(), x.pair(), y.pair()), ignore_zero_sign=False) == 0
+    assert nodd >= len(range(0, n, 6)) - len([k for k in range(0, n, 6) if k % 11 == 2])
+    for a_val in (ZERO, v(ones, -5, -1), v(top, 0)):
+        a = pack([a_val], pb)
+        got, _ = run_real(emul_real, pb, a.pair(), x.pair(), y.pair())
+        assert O.mismatches(got, O.axpy_real(pb, a.pair(), x.pair(), y.pair()), ignore_zero_sign=False) == 0
