@@ -438,7 +438,7 @@ def batch_config(with_cpu, n_tasks=10000, seed=5, check=50):
 
 def side_rows():
     """Rows beside the path: complex GEMM pb 256 1024^3 (per-op and exact),
-    division and square root (pb 256 and 1024, n = 2^18), run-time precision
+    complex GEMV pb 512 4096^2, division and square root (pb 256 and 1024, n = 2^18), run-time precision
     AXPY (1024 and 1000 bits, n = 16384)."""
     import mpc_cuda_amd as M
     out = {}
@@ -451,6 +451,13 @@ def side_rows():
                                  "perop_roofline": valu_roof(4 * WP(pb) * d ** 3, ms_p),
                                  "exact_MP_ops_per_s": 2 * d ** 3 / (ms_e * 1e-3)}
     del A, Bm
+    pb, m = 512, 4096
+    A = M.device_random(pb, m * m, 521, -4, 4, cplx=True)
+    xv = M.device_random(pb, m, 522, -4, 4, cplx=True)
+    ms_g = _timed(lambda: M.gemv(A, xv, m, m), 2)
+    out["gemv_complex_pb512_4096x4096"] = {"ms": ms_g, "value": 2 * m * m / (ms_g * 1e-3), "unit": "MP ops/s",
+                                           "roofline": valu_roof(4 * WP(pb) * m * m, ms_g)}
+    del A, xv
     for pb in (256, 1024):
         n = 1 << 18
         a = M.device_random(pb, n, 601, -30, 30)

@@ -116,6 +116,16 @@ int mpgfx_gemv_real (int pb, int64_t m, int64_t k,
                      const uint64_t *x_limbs, const int64_t *x_se,
                      uint64_t *y_limbs, int64_t *y_se, void *stream);
 
+/* Complex per-operation GEMV: y[i] = fold over k = 0..k-1 of acc = acc +
+ * A[i,k]*x[k], acc starting at +0, every product and sum MPC_RNDNN (each
+ * component rounded once).  A is an m x k complex MP matrix, x a complex MP
+ * array of k, y a complex MP array of m.  Replaces demos/matvec_mpc.cu:42-67
+ * (cdot_row, matvec_kernel) on the fixed type. */
+int mpgfx_gemv_complex (int pb, int64_t m, int64_t k,
+                        const uint64_t *A_limbs, const int64_t *A_se,
+                        const uint64_t *x_limbs, const int64_t *x_se,
+                        uint64_t *y_limbs, int64_t *y_se, void *stream);
+
 /* Bytes of device workspace mpgfx_gemm_real needs for this shape (0 for the
  * per-operation mode).  No counterpart in the reference (its kernels need no
  * workspace). */

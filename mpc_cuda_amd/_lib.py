@@ -31,6 +31,7 @@ SIGNATURES = {
     "mpgfx_axpy_real": (_i, [_i, _l] + [_p] * 8 + [_p]),
     "mpgfx_axpy_complex": (_i, [_i, _l] + [_p] * 8 + [_p]),
     "mpgfx_gemv_real": (_i, [_i, _l, _l] + [_p] * 6 + [_p]),
+    "mpgfx_gemv_complex": (_i, [_i, _l, _l] + [_p] * 6 + [_p]),
     "mpgfx_gemm_workspace_size": (_i, [_i, _i, _l, _l, _l, ctypes.POINTER(ctypes.c_size_t)]),
     "mpgfx_gemm_real": (_i, [_i, _i, _l, _l, _l] + [_p] * 6 + [_p, ctypes.c_size_t, _p]),
     "mpgfx_gemm_complex": (_i, [_i, _l, _l, _l] + [_p] * 6 + [_p]),

@@ -113,6 +113,17 @@ extern "C" int mpgfx_gemv_real (int pb, int64_t m, int64_t k, const uint64_t *Al
   return o->gemv (m, k, Al, Ase, xl, xse, yl, yse, (hipStream_t) stream);
 }
 
+extern "C" int mpgfx_gemv_complex (int pb, int64_t m, int64_t k, const uint64_t *Al, const int64_t *Ase,
+                                   const uint64_t *xl, const int64_t *xse, uint64_t *yl, int64_t *yse, void *stream)
+{
+  if (m < 0 || k < 0) return MPGFX_EINVAL;
+  GET_OPS (o);
+  if (m == 0) return MPGFX_OK;
+  NEED (yl && yse);
+  if (k > 0) NEED (Al && Ase && xl && xse);
+  return o->gemm_cplx (m, 1, k, Al, Ase, xl, xse, yl, yse, (hipStream_t) stream);
+}
+
 extern "C" int mpgfx_gemm_workspace_size (int pb, int mode, int64_t m, int64_t n, int64_t k, size_t *bytes)
 {
   if (!bytes || m < 0 || n < 0 || k < 0) return MPGFX_EINVAL;

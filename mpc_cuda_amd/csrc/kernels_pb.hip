@@ -819,6 +819,82 @@ __global__ __launch_bounds__ (64) void k_gemv_fused (i64 m, i64 k, const u64 *__
     }
 }
 
+/* Complex GEMV y_i = fold_k (acc + RNDN_c(A[i,k] x[k])), per-op in k order
+ * (each product component rounded once, cu_cmul; the complex addition
+ * componentwise, cu_cadd), with the products inside the cooperative fold as in
+ * k_gemv_fused.  Complex addition is componentwise, so the real and the
+ * imaginary parts of a row are two independent real folds: a wavefront holds
+ * 64/G chains, chain c the part c & 1 of row c >> 1.  Per block of G steps
+ * every lane computes one product component, RNDN(ar u -/+ ai v) for its
+ * row, part and step (core::fmm, the operands chosen per part), and writes it
+ * as a record to LDS; the G steps of the block fold their chain's records.
+ * Precisions whose group is the whole wavefront (G = 64) use k_gemm_cplx. */
+constexpr bool FGC = FG_G <= 32;
+
+__global__ __launch_bounds__ (64) void k_gemv_cplx_fused (i64 m, i64 k, const u64 *__restrict__ Al,
+                                                          const i64 *__restrict__ Ase, const u64 *__restrict__ xl,
+                                                          const i64 *__restrict__ xse, u64 *__restrict__ yl,
+                                                          i64 *__restrict__ yse)
+{
+  if constexpr (FGC)
+    {
+      namespace cd = mpgfx::coopd;
+      constexpr int G = FG_G, CH = 64 / G, RC = CH / 2, REC = NL + 1;
+      __shared__ u64 lds[CH * G * REC];
+      const cd::DevLanes w;
+      const int lane = (int) threadIdx.x, ch = lane / G, r = ch >> 1, p = ch & 1, j = lane % G;
+      const int jj = j < 1 ? 0 : (j > NL ? NL - 1 : j - 1);
+      const i64 i = (i64) blockIdx.x * RC + r, ii = i < m ? i : m - 1;
+      const i64 nA = m * k, oA = (i64) NL * nA, oX = (i64) NL * k;
+      cd::cv acc = cd::zero ();
+      /* operands of this lane's first product */
+      i64 kk = j < k ? j : k - 1;
+      cu_freal<PB> ar = ld<PB> (Al, Ase, nA, ii * k + kk), ai = ld<PB> (Al + oA, Ase + nA, nA, ii * k + kk);
+      cu_freal<PB> xr = ldc<PB> (xl, xse, k, kk), xi = ldc<PB> (xl + oX, xse + k, k, kk);
+      for (i64 s0 = 0; s0 < k; s0 += G)
+        {
+          const cu_freal<PB> pr = cu_fp::core::fmm<PB> (ar, p ? xi : xr, ai, p ? xr : xi, p == 0);
+          u64 *rec = lds + (ch * G + j) * REC;
+#pragma unroll
+          for (int l = 0; l < NL; l++) rec[l] = pr.m[l];
+          rec[NL] = (u64) (((i64) pr.exp << 1) | (pr.sign < 0 ? 1 : 0));
+          __syncthreads ();
+          /* this block's records of the chain (lane j' holds limb j'-1) */
+          cd::raw q[G];
+#pragma unroll
+          for (int d = 0; d < G; d++)
+            {
+              const u64 *src = lds + (ch * G + d) * REC;
+              q[d].m = src[jj];
+              q[d].w = (i64) src[NL];
+            }
+          /* the next block's operands, loaded while this block folds */
+          const i64 sn = s0 + G + j;
+          kk = sn < k ? sn : k - 1;
+          ar = ld<PB> (Al, Ase, nA, ii * k + kk);
+          ai = ld<PB> (Al + oA, Ase + nA, nA, ii * k + kk);
+          xr = ldc<PB> (xl, xse, k, kk);
+          xi = ldc<PB> (xl + oX, xse + k, k, kk);
+          const i64 steps = k - s0;
+#pragma unroll
+          for (int d = 0; d < G; d++)
+            {
+              const cd::cv nv = cd::add<PB> (w, acc, cd::decode<PB> (w, q[d]));
+              const bool live = d < steps;
+              acc.m = live ? nv.m : acc.m;
+              acc.exp = live ? nv.exp : acc.exp;
+              acc.sign = live ? nv.sign : acc.sign;
+            }
+          __syncthreads ();
+        }
+      cval out;
+      out.m = acc.m;
+      out.exp = acc.exp;
+      out.sign = acc.sign;
+      if (i < m) coop_store<PB> (out, yl + (i64) p * NL * m, yse + (i64) p * m, m, i);
+    }
+}
+
 /* AoS (reference struct layout) <-> SoA, one lane per real value */
 __global__ void k_pack (i64 n, int cplx, const cu_freal<PB> *__restrict__ aos, u64 *__restrict__ l,
                         i64 *__restrict__ se)
@@ -925,6 +1001,14 @@ int gemm_perop (i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *B
 int gemm_cplx (i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl, const i64 *Bse, u64 *Cl,
                i64 *Cse, hipStream_t s)
 {
+  if (FGC && n == 1 && k > 0)
+    {
+      /* a GEMV: the products inside the cooperative fold */
+      constexpr i64 RC = 32 / FG_G;
+      hipLaunchKernelGGL (k_gemv_cplx_fused, dim3 ((unsigned) ((m + RC - 1) / RC)), dim3 (64), 0, s, m, k, Al, Ase, Bl,
+                          Bse, Cl, Cse);
+      return launch_status ("gemv_complex");
+    }
   i64 tiles = m * ((n + TPB - 1) / TPB);
   unsigned g = (unsigned) (tiles > 65536 ? 65536 : tiles);
   hipLaunchKernelGGL (k_gemm_cplx, dim3 (g), dim3 (TPB), 0, s, m, n, k, Al, Ase, Bl, Bse, Cl, Cse);

@@ -143,14 +143,18 @@ def axpy(a, x, y, out=None):
 
 
 def gemv(A, x, m, k, out=None):
-    """y = A x with per-operation rounding in k order (A: m*k values)."""
+    """y = A x with per-operation rounding in k order (A: m*k values; real,
+    or complex with MPC_RNDNN products and sums)."""
     _dev(A), _dev(x)
     _same(A.pb, x)
-    if A.n != m * k or x.n != k or A.cplx or x.cplx:
+    if A.n != m * k or x.n != k or A.cplx != x.cplx:
         raise ValueError("shape mismatch")
-    out = _out(out, A.pb, m, False, A)
+    out = _out(out, A.pb, m, A.cplx, A)
     L = _lib.lib()
-    _lib.check("mpgfx_gemv_real", L.mpgfx_gemv_real(A.pb, m, k, *A.ptrs(), *x.ptrs(), *out.ptrs(), _stream()))
+    if A.cplx:
+        _lib.check("mpgfx_gemv_complex", L.mpgfx_gemv_complex(A.pb, m, k, *A.ptrs(), *x.ptrs(), *out.ptrs(), _stream()))
+    else:
+        _lib.check("mpgfx_gemv_real", L.mpgfx_gemv_real(A.pb, m, k, *A.ptrs(), *x.ptrs(), *out.ptrs(), _stream()))
     return out
 
 
