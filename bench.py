@@ -70,10 +70,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-configs", action="store_true", help="headline only")
-    ap.add_argument("--driver", choices=["axpy", "exact", "gemv", "gemm"], default=None,
+    ap.add_argument("--driver", choices=["axpy", "axpyr", "exact", "gemv", "gemvc", "gemm"], default=None,
                     help="only run a few launches of one kernel (for rocprofv3 --pmc): the headline AXPY "
-                         "plus calibration copies, the exact GEMM at 1024^3, the GEMV at 4096^2, or the "
-                         "per-op GEMM at 1024^3")
+                         "plus calibration copies, the real AXPY pb 2048 n = 2^20, the exact GEMM at 1024^3, "
+                         "the GEMV at 4096^2 (real or complex), or the per-op GEMM at 1024^3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -203,7 +203,7 @@ def cpu_headline(a, x, y, z):
     out = {}
     if O.cpu_available():
         _, ms1 = O.cpu_axpy(PB, ap, xp, yp, True)
-        reps = max(1, min(200, int(3000.0 / max(ms1, 1e-3))))
+        reps = max(1, min(2000, int(10000.0 / max(ms1, 1e-3))))
         res, ms = O.cpu_axpy(PB, ap, xp, yp, True, reps=reps)
         out = dict(_cpu_common())
         out.update({"value": 2.0 * N_ELEM / (ms * 1e-3),
@@ -490,6 +490,8 @@ def profile_driver(which):
     """Launches for the rocprofv3 passes summarised under profiles/:
     "axpy": three headline AXPYs on device-generated inputs, then three 512 MiB
     device copies whose FETCH_SIZE / WRITE_SIZE calibrate the counters;
+    "axpyr": three real AXPYs pb 2048, n = 2^20; "gemvc": two complex GEMVs pb
+    512 4096^2;
     "exact": three exact GEMMs pb 256 1024^3 (exponents in [-4, 4]);
     "gemv": three per-op GEMVs pb 512 4096^2; "gemm": two per-op GEMMs pb 256
     1024^3."""
@@ -505,6 +507,19 @@ def profile_driver(which):
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
+    elif which == "axpyr":
+        pb, n = 2048, 1 << 20
+        a = M.device_random(pb, 1, 101 + pb, -8, 8)
+        x = M.device_random(pb, n, 102 + pb, -8, 8)
+        y = M.device_random(pb, n, 103 + pb, -8, 8)
+        for _ in range(3):
+            M.axpy(a, x, y)
+    elif which == "gemvc":
+        m = k = 4096
+        A = M.device_random(512, m * k, 521, -4, 4, cplx=True)
+        x = M.device_random(512, k, 522, -4, 4, cplx=True)
+        for _ in range(2):
+            M.gemv(A, x, m, k)
     elif which == "exact":
         d = 1024
         A = M.device_random(256, d * d, 401, -4, 4)

@@ -44,16 +44,17 @@
  * entry (0 <= d - u <= ND).
  *
  * Work per wavefront: chunks of 32 elements; lane L = 2e + p (element e, part
- * p: 0 real, 1 imaginary).  Per chunk: (1) lane L loads x_p[e], recodes it
- * into digits and writes them to LDS; (2) the B tiles (x digits of both parts,
+ * p: 0 real, 1 imaginary).  Per chunk: (1) lane L recodes x_p[e] (staged in
+ * LDS) into digits in place; (2) the B tiles (x digits of both parts,
  * two 16-element column tiles, every K block) are read into registers; (3)
  * pass q = 0 multiplies by a_r, pass q = 1 by a_i: for each x part p' and
  * column tile the formed tiles are accumulated, each lane turns its four digit
  * sums into one signed 64-bit word value and writes it to LDS for the lane
  * that carries that product -- lane 2e + (p' ^ q), so that lane 2e receives
  * a_r x_r then a_i x_i (the real part) and lane 2e+1 receives a_r x_i then
- * a_i x_r (the imaginary part); the word values pass through LDS in two groups
- * of output tiles, each carried before the next is formed; (4) each lane
+ * a_i x_r (the imaginary part); the word values pass through LDS in groups of
+ * three output tiles, each carried before the next is formed (while the next
+ * chunk's x is copied into the staging rows); (4) each lane
  * carries its product's word values into words, adding a_q's words at offset
  * KW, and normalises the window; (5) core::add_round rounds the two terms once
  * (RNDN(p0 -/+ p1)), the lanes it leaves open take the exact products, then
@@ -61,7 +62,7 @@
  *
  * LDS per workgroup (8 wavefronts, two per SIMD): the A tiles of a_r and a_i
  * (2 x NF x 1 KB, built once per workgroup from a's digits), a's words, and
- * per wavefront the word-value buffer (which first holds the x digits).
+ * per wavefront the x staging rows and the word-value buffer.
  */
 #ifndef MPGFX_AXPY_MFMA_H
 #define MPGFX_AXPY_MFMA_H

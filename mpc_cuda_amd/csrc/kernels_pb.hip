@@ -217,25 +217,28 @@ __global__ __launch_bounds__ (TPB) void k_axpy_cplx_pipe (i64 nchunks, i64 n, co
 /* The complex AXPY with its products on INT8 MFMA (axpy_mfma.h): whole
  * 32-element chunks, one chunk per wavefront at a time, AXM_WAVES wavefronts
  * per workgroup sharing the A tiles and the words of a_r and a_i in LDS, two
- * wavefronts per SIMD.  Per wavefront, LDS holds a word-value buffer through
- * which a product's words pass in two groups of output tiles (it first holds
- * the chunk's recoded x digits, from which the B tiles are read). */
+ * wavefronts per SIMD.  Per wavefront, LDS holds the x staging rows of a
+ * chunk (limb-major, as in HBM: the digits are recoded in place and the B
+ * tiles read from there, after which the next chunk's x is copied in with
+ * global_load_lds while this one computes) and a word-value buffer through
+ * which a product's words pass in groups of TG output tiles. */
 constexpr bool AXM = PB == 1024;
 constexpr int AXM_WAVES = 8;
 constexpr int AXM_TPB = 64 * AXM_WAVES;
 namespace axm {
 typedef axmf::cfg<AXM ? PB : 1024> C;
-constexpr int TG = (C::NT + 1) / 2;                    /* output tiles per pass through LDS */
+constexpr int TG = 3;                                  /* output tiles per pass through LDS */
 constexpr int NG = (C::NT + TG - 1) / TG;
 constexpr int PBS = 64 + 2;                            /* word-value row: 64 lanes, padded */
 constexpr int PBUF = 4 * TG * PBS * 8;
-constexpr int DGS = C::KW + 4;                         /* digit row stride (words), 16-byte rows */
-constexpr int WAVE = PBUF;
+constexpr int STG_ROWS = 2 * NL + 4;                   /* x limbs (Re, Im), both sign/exponent rows, padding */
+constexpr int STG = STG_ROWS * 256;
+constexpr int WAVE = STG + PBUF;
 constexpr int ATAB = 2 * C::NF * 64 * 16;              /* A tiles [q][tile][lane], 16 bytes a lane */
 constexpr int TPS = 2 * C::TPW * 4;                    /* padded digit strings of a_r, a_i */
 constexpr int AWS = 2 * C::KW * 4;                     /* words of a_r, a_i */
 constexpr int LDSB = AXM ? ATAB + AXM_WAVES * WAVE + TPS + AWS : 16;
-static_assert (!AXM || (64 * DGS * 4 <= PBUF && C::KW == cu_freal<PB>::KW && cu_freal<PB>::SB == 0), "layout");
+static_assert (!AXM || (C::KW == cu_freal<PB>::KW && cu_freal<PB>::SB == 0), "layout");
 static_assert (LDSB <= 160 * 1024, "LDS");
 }
 
@@ -360,40 +363,49 @@ __global__ __launch_bounds__ (AXM_TPB) void k_axpy_cplx_mfma (i64 nchunks, i64 n
       });
       __syncthreads ();
 
-      const int e = L >> 1, p = L & 1, col = L & 15, g = L >> 4;
+      const int e = L >> 1, p = L & 1, col = L & 15, g = L >> 4, quarter = L >> 4, lane_e = 2 * (L & 15);
       /* a's sign/exponent words */
       const i64 sea[2] = {ase[0], ase[1]};
-      i64 *const pw = (i64 *) (lds + ATAB + w * WAVE);
-      u32 *const dg = (u32 *) pw;
+      unsigned char *const stg = lds + ATAB + w * WAVE;
+      i64 *const pw = (i64 *) (stg + STG);
       const i64 wstride = (i64) gridDim.x * AXM_WAVES;
+      auto copy_in = [&] (i64 cc) {
+        const u64 *src = xl + (i64) quarter * n + cc * 32 + lane_e;
+#pragma unroll
+        for (int qq = 0; qq < 2 * NL / 4; qq++)
+          {
+            __builtin_amdgcn_global_load_lds (src, (lds_ptr) (stg + qq * 1024), 16, 0, 0);
+            src += 4 * n;
+            asm volatile ("" : "+v"(src));
+          }
+        const i64 *ssrc = xse + (i64) (quarter & 1) * n + cc * 32 + lane_e;
+        __builtin_amdgcn_global_load_lds (ssrc, (lds_ptr) (stg + (2 * NL / 4) * 1024), 16, 0, 0);
+      };
+      if ((i64) blockIdx.x * AXM_WAVES + w < nchunks) copy_in ((i64) blockIdx.x * AXM_WAVES + w);
+      __builtin_amdgcn_s_waitcnt (0);
       for (i64 c = (i64) blockIdx.x * AXM_WAVES + w; c < nchunks; c += wstride)
         {
-          const i64 i = c * 32 + e;
+          const i64 i = c * 32 + e, cn = c + wstride;
           /* this lane's x part (one per-lane pointer stepped by a plane, kept
            * opaque so that the compiler holds no address per plane), both
            * parts' sign/exponent words */
-          const i64 se0 = xse[i], se1 = xse[n + i];
+          const i64 se0 = *(const i64 *) (stg + (2 * NL) * 256 + e * 8);
+          const i64 se1 = *(const i64 *) (stg + (2 * NL + 1) * 256 + e * 8);
           {
             u32 xw[KW], dgw[KW];
-            const u64 *xp = xl + p * o + i;
 #pragma unroll
             for (int k = 0; k < NL; k++)
               {
-                const u64 v = __builtin_nontemporal_load (xp);
-                xp += n;
-                asm volatile ("" : "+v"(xp));
+                const u64 v = *(const u64 *) (stg + (p * NL + k) * 256 + e * 8);
                 xw[2 * k] = (u32) v;
                 xw[2 * k + 1] = (u32) (v >> 32);
               }
             axmf::recode<KW> (dgw, xw);
-            u32 *row = dg + (p * 32 + e) * DGS;
 #pragma unroll
-            for (int j = 0; j < KW / 4; j++)
-              *(v4i *) (row + 4 * j) = (v4i) {(int) dgw[4 * j], (int) dgw[4 * j + 1], (int) dgw[4 * j + 2],
-                                              (int) dgw[4 * j + 3]};
+            for (int k = 0; k < NL; k++)
+              *(u64 *) (stg + (p * NL + k) * 256 + e * 8) = ((u64) dgw[2 * k + 1] << 32) | dgw[2 * k];
           }
           wave_lds_sync ();
-          /* B tiles: x digits of both parts, both column tiles, every K block */
           v4i bf[2][2][KB];
 #pragma unroll
           for (int pp = 0; pp < 2; pp++)
@@ -401,8 +413,14 @@ __global__ __launch_bounds__ (AXM_TPB) void k_axpy_cplx_mfma (i64 nchunks, i64 n
             for (int nt = 0; nt < 2; nt++)
 #pragma unroll
               for (int b = 0; b < KB; b++)
-                bf[pp][nt][b] = *(const v4i *) (dg + (pp * 32 + nt * 16 + col) * DGS + 16 * b + 4 * g);
+                {
+                  const unsigned char *r0 = stg + (pp * NL + 8 * b + 2 * g) * 256 + (nt * 16 + col) * 8;
+                  const u64 lo = *(const u64 *) r0, hi = *(const u64 *) (r0 + 256);
+                  bf[pp][nt][b] = (v4i) {(int) (u32) lo, (int) (u32) (lo >> 32), (int) (u32) hi, (int) (u32) (hi >> 32)};
+                }
           wave_lds_sync ();
+          __builtin_amdgcn_s_waitcnt (0xC07F);
+          if (cn < nchunks) copy_in (cn);
           axmf::term<P> tm[2];
           cu_freal<P> yv;
           sfor<0, 2> ([&] (auto Qc) {
@@ -458,6 +476,7 @@ __global__ __launch_bounds__ (AXM_TPB) void k_axpy_cplx_mfma (i64 nchunks, i64 n
           /* the exact products for the lanes the bound leaves open (rare) */
           if (__builtin_amdgcn_ballot_w64 (amb) != 0 && amb) prod = axm_exact (al, ase, xl, xse, n, i, p);
           const cu_freal<P> z = cu_fp::cu_fadd<P> (prod, yv);
+          __builtin_amdgcn_s_waitcnt (0);
           {
             u64 *zp = zl + p * o + i;
 #pragma unroll

@@ -148,12 +148,14 @@ long run (int64_t n, const uint64_t *al, const int64_t *ase, const uint64_t *xl,
               term<PB> &t = tm[L][q];
               const bool zero = aq.is_zero () || (sew >> 1) == (int64_t) CU_FREAL_ZERO_EXP;
               t.exp = (int64_t) aq.exp + (sew >> 1);
-              /* in two halves of the output tiles, as the kernel carries them */
+              /* in groups of three output tiles, as the kernel carries them */
               carry<PB> cs;
               cs.c = 0;
-              constexpr int T0 = (NT + 1) / 2;
-              sweep_range<PB, 0, 4 * T0> (cs, [&] (int ww) { return pw[ww * 64 + L]; }, [&] (int j) { return (q ? aiw : arw)[j]; });
-              sweep_range<PB, 4 * T0, 4 * NT> (cs, [&] (int ww) { return pw[(4 * T0 + ww) * 64 + L]; }, [&] (int j) { return (q ? aiw : arw)[j]; });
+              auto aw = [&] (int j) { return (q ? aiw : arw)[j]; };
+              cu_fp::core::sfor<0, (NT + 2) / 3> ([&] (auto Gc) {
+                constexpr int TA = 3 * decltype (Gc)::value, TB = TA + 3 < NT ? TA + 3 : NT;
+                sweep_range<PB, 4 * TA, 4 * TB> (cs, [&] (int ww) { return pw[(4 * TA + ww) * 64 + L]; }, aw);
+              });
               sweep_finish<PB> (t, cs);
               t.zero = zero;
               t.sign = zero ? 1 : aq.sign * ((sew & 1) ? -1 : 1);
