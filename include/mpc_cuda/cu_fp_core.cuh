@@ -104,6 +104,12 @@ CUFP_INL bool any_lane (bool c)
 #endif
 }
 
+/* Placed first in a branch that only rare data takes: the empty volatile
+ * statement cannot be executed speculatively, so the compiler keeps the
+ * branch (a wave-uniform jump) instead of turning its body into selects that
+ * every lane would execute every time. */
+#define CUFP_RARE() asm volatile ("")
+
 CUFP_INL int clz32 (u32 x)
 {
   return x ? __builtin_clz (x) : 32;
@@ -300,8 +306,15 @@ CUFP_INL u32 round_top (u32 (&K)[KW], const u32 (&D)[LEN], u32 s, u32 xst)
     }
   u32 inc = rbit & ((sticky != 0) | (K[0] & 1u));
   u32 c = inc;
-  CUFP_UNROLL
-  for (int j = 0; j < KW; j++) K[j] = addc (K[j], 0u, c);
+  K[0] = addc (K[0], 0u, c);
+  /* the increment carries past the lowest kept word only when that word was
+   * all ones: a chain of dependent add-with-carry steps just for that case */
+  if (any_lane (c != 0))
+    {
+      CUFP_RARE ();
+      CUFP_UNROLL
+      for (int j = 1; j < KW; j++) K[j] = addc (K[j], 0u, c);
+    }
   /* carry-out: K wrapped to 0; the value is 2^PB = 2^(PB-1) * 2 */
   K[KW - 1] |= c << 31;
   return c;
@@ -446,6 +459,7 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
     const bool neg = sub && (D[L - 1] >> 31) != 0;
     if (any_lane (neg))
       {
+        CUFP_RARE ();
         u32 cn = 1;
         CUFP_UNROLL
         for (int i = 0; i < L; i++)
@@ -473,6 +487,7 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
   bool zres = false;
   if (any_lane (E[L - 2] == 0))
     {
+      CUFP_RARE ();
       lz = 0;
       bool seen = false;
       CUFP_UNROLL
