@@ -122,10 +122,11 @@ CUFP_INL int clz64 (u64 x)
 /* ---------------------------------------------------------------------------
  * Exact product p[2W] = a[W] * b[W] of two W-word significands.
  *
- * W < 16 (pb < 512): schoolbook by rows on 32-bit words (one 32x32+64
- * multiply-add per word product plus the carry add); at these sizes the limb
- * split and recombination below cost more than they save (measured on the
- * pb 256 GEMM and AXPY).
+ * W < 16 (pb < 512): schoolbook on 32-bit words, by columns on the device
+ * (one 32x32+64 multiply-add per word product, its carry out counted with one
+ * add-with-carry; mul_words_cols) and by rows on the host; at these sizes the
+ * limb split and recombination below cost more than they save (measured on
+ * the pb 256 GEMM and AXPY).
  * W >= 16: the operands are re-cut into LB-bit limbs (LB = 29, or 28 for the
  * widest precisions) and the product is accumulated by columns: a column of
  * up to 63 (255) limb products plus the incoming carry cannot overflow 64 bits,
@@ -266,13 +267,55 @@ CUFP_INL void mul_words_short (u32 (&p)[H], const u32 (&a)[W], const u32 (&b)[W]
   });
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+/* acc += a*b on 64 bits, the carry out of that addition added to ovf: the
+ * carry mask v_mad_u64_u32 writes is the add-with-carry's carry-in (the
+ * compiler exposes neither, hence the two instructions written out; s_nop 1
+ * gives the mask its wait states before it is read) */
+__device__ __forceinline__ void mac_carry (u64 &acc, u32 &ovf, u32 a, u32 b)
+{
+  u64 t, cm, cm2;
+  asm ("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(t), "=s"(cm) : "v"(a), "v"(b), "v"(acc));
+  asm ("s_nop 1\n\tv_addc_co_u32_e64 %0, %1, %2, 0, %3" : "=v"(ovf), "=s"(cm2) : "v"(ovf), "s"(cm));
+  acc = t;
+}
+
+/* W < 16 on the device: product scanning by columns.  Every word product is
+ * one v_mad_u64_u32 into the column's 64-bit sum, its carry out counted in
+ * the column's overflow word; the column's low word is emitted and the rest,
+ * (sum >> 32) + overflow * 2^32, starts the next column.  No zero-extended
+ * addends are built (the row form needs two moves and a 64-bit add per word
+ * product for them). */
+template <int W>
+__device__ __forceinline__ void mul_words_cols (u32 (&p)[2 * W], const u32 (&a)[W], const u32 (&b)[W])
+{
+  u64 acc = 0;
+  sfor<0, 2 * W - 1> ([&] (auto K) {
+    constexpr int k = decltype (K)::value;
+    constexpr int i0 = k >= W ? k - W + 1 : 0, i1 = k < W ? k + 1 : W;
+    u32 ovf = 0;
+    sfor<i0, i1> ([&] (auto I) {
+      constexpr int i = decltype (I)::value;
+      mac_carry (acc, ovf, a[i], b[k - i]);
+    });
+    p[k] = (u32) acc;
+    acc = (acc >> 32) | ((u64) ovf << 32);
+  });
+  p[2 * W - 1] = (u32) acc;
+}
+#endif
+
 template <int W>
 CUFP_INL void mul_words (u32 (&p)[2 * W], const u32 (&a)[W], const u32 (&b)[W])
 {
   if constexpr (W >= 16)
     mul_words_limbs<W> (p, a, b);
   else
+#if defined(__HIP_DEVICE_COMPILE__)
+    mul_words_cols<W> (p, a, b);
+#else
     mul_words_rows<W> (p, a, b);
+#endif
 }
 
 /* ---------------------------------------------------------------------------
@@ -471,23 +514,42 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
       }
   }
 
-  /* carry into the top word: shift right by one, dropped bit -> sticky */
-  u32 cy = D[L - 1] & 1u;
+  /* Normalise into E (L-1 words, top bit in E[L-2]): with a carry into the
+   * top word the window moves right by one bit (the dropped bit goes to the
+   * sticky), otherwise left by the leading zeros lz. */
+  const u32 cy = D[L - 1] & 1u;
   st |= cy & D[0];
   u32 E[L - 1];
-  CUFP_UNROLL
-  for (int i = 0; i < L - 1; i++)
-    {
-      u32 hi = D[i + 1];
-      E[i] = cy ? fshr (hi, D[i], 1u) : D[i];
-    }
-  /* leading zeros of E (L-1 words); a full scan only when some lane has
-   * cancelled a whole word */
-  u32 lz = (u32) clz32 (E[L - 2]);
+  u32 lz;
   bool zres = false;
-  if (any_lane (E[L - 2] == 0))
+  if (!any_lane (cy == 0u && D[L - 2] == 0u))
     {
+      /* no lane cancelled a whole word: one funnel shift per word, right by
+       * cy from the pairs (D[i+1]:D[i]) when there is a carry or no leading
+       * zero, else right by 32 - lz from the pairs (D[i]:D[i-1]) */
+      lz = cy ? 0u : (u32) clz32 (D[L - 2]);
+      const bool up = cy != 0u || lz == 0u;
+      const u32 sa = up ? cy : 32u - lz;
+      u32 lo = up ? D[0] : 0u;
+      CUFP_UNROLL
+      for (int i = 0; i < L - 1; i++)
+        {
+          const u32 hi = up ? D[i + 1] : D[i];
+          E[i] = fshr (hi, lo, sa);
+          lo = hi;
+        }
+    }
+  else
+    {
+      /* some lane cancelled a whole word (rare): carry shift, a full scan
+       * of the leading zeros, word steps, then the bit shift */
       CUFP_RARE ();
+      CUFP_UNROLL
+      for (int i = 0; i < L - 1; i++)
+        {
+          u32 hi = D[i + 1];
+          E[i] = cy ? fshr (hi, D[i], 1u) : D[i];
+        }
       lz = 0;
       bool seen = false;
       CUFP_UNROLL
@@ -498,11 +560,7 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
           seen = seen || (E[i] != 0);
         }
       zres = !seen;
-    }
-  u32 wl = lz >> 5, bl = lz & 31u;
-  if (wl != 0)
-    {
-      /* cancellation of >= 32 bits: word-level left shift (rare) */
+      u32 wl = lz >> 5, bl = lz & 31u;
       CUFP_UNROLL
       for (int k = 0; (1 << k) <= L - 1; k++)
         {
@@ -516,12 +574,12 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
               E[i] = on ? v : E[i];
             }
         }
-    }
-  CUFP_UNROLL
-  for (int i = L - 2; i >= 0; i--)
-    {
-      u32 lo = (i >= 1) ? E[i - 1] : 0u;
-      E[i] = fshl (E[i], lo, bl);
+      CUFP_UNROLL
+      for (int i = L - 2; i >= 0; i--)
+        {
+          u32 lo = (i >= 1) ? E[i - 1] : 0u;
+          E[i] = fshl (E[i], lo, bl);
+        }
     }
   r.amb = false;
   if constexpr (ERRB > 0)
