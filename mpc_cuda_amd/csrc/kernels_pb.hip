@@ -731,6 +731,69 @@ __global__ __launch_bounds__ (TPB) void k_gemm_perop (i64 m, i64 n, i64 k, const
     }
 }
 
+/* The per-op GEMM on 16 x 16 output tiles, one lane per C element, the
+ * operands staged through LDS: per k block of 16, the workgroup copies
+ * A[i0..i0+15, k0..k0+15] and B[k0..k0+15, j0..j0+15] (coalesced 16-value row
+ * segments of every limb plane) into LDS once, and each lane reads its A
+ * value (shared by the 16 lanes of its row) and its B value (shared by the 16
+ * lanes of its column) from there: the global reads per step fall from one
+ * value per lane to one eighth. */
+constexpr int GT = 16;
+__global__ __launch_bounds__ (256) void k_gemm_perop_tiled (i64 m, i64 n, i64 k, const u64 *__restrict__ Al,
+                                                           const i64 *__restrict__ Ase, const u64 *__restrict__ Bl,
+                                                           const i64 *__restrict__ Bse, u64 *__restrict__ Cl,
+                                                           i64 *__restrict__ Cse)
+{
+  constexpr int NP = cu_freal<PB>::N + 1;                /* limb planes and the sign/exponent plane */
+  __shared__ u64 sa[NP][GT * GT], sb[NP][GT * GT];
+  const int r = threadIdx.x >> 4, c = threadIdx.x & 15;
+  const i64 tm = (m + GT - 1) / GT, tn = (n + GT - 1) / GT;
+  const i64 nA = m * k, nB = k * n, nC = m * n;
+  for (i64 t = blockIdx.x; t < tm * tn; t += gridDim.x)
+    {
+      const i64 i0 = (t / tn) * GT, j0 = (t % tn) * GT;
+      const i64 i = i0 + r, j = j0 + c;
+      cu_freal<PB> acc;
+      acc.set_zero ();
+      for (i64 k0 = 0; k0 < k; k0 += GT)
+        {
+          /* stage: lane (r, c) copies A[i0 + r, k0 + c] and B[k0 + r, j0 + c] */
+          {
+            const bool va = (i < m) && (k0 + c < k), vb = (k0 + r < k) && (j < n);
+            const i64 ia = va ? i * k + k0 + c : 0, ib = vb ? (k0 + r) * n + j : 0;
+#pragma unroll
+            for (int q = 0; q < NP - 1; q++)
+              {
+                sa[q][threadIdx.x] = va ? Al[(i64) q * nA + ia] : 0ull;
+                sb[q][threadIdx.x] = vb ? Bl[(i64) q * nB + ib] : 0ull;
+              }
+            sa[NP - 1][threadIdx.x] = va ? (u64) Ase[ia] : 0ull;
+            sb[NP - 1][threadIdx.x] = vb ? (u64) Bse[ib] : 0ull;
+          }
+          __syncthreads ();
+          const int kb = (k - k0 < GT) ? (int) (k - k0) : GT;
+          for (int kk = 0; kk < kb; kk++)
+            {
+              cu_freal<PB> a, b;
+#pragma unroll
+              for (int q = 0; q < NP - 1; q++)
+                {
+                  a.m[q] = sa[q][r * GT + kk];
+                  b.m[q] = sb[q][kk * GT + c];
+                }
+              const i64 wa = (i64) sa[NP - 1][r * GT + kk], wb = (i64) sb[NP - 1][kk * GT + c];
+              a.sign = (wa & 1) ? -1 : 1;
+              a.exp = (long) (wa >> 1);
+              b.sign = (wb & 1) ? -1 : 1;
+              b.exp = (long) (wb >> 1);
+              acc = cu_fp::cu_fadd<PB> (acc, cu_fp::cu_fmul<PB> (a, b));
+            }
+          __syncthreads ();
+        }
+      if (i < m && j < n) st<PB> (acc, Cl, Cse, nC, i * n + j);
+    }
+}
+
 __global__ __launch_bounds__ (TPB) void k_gemm_cplx (i64 m, i64 n, i64 k, const u64 *__restrict__ Al,
                                                      const i64 *__restrict__ Ase, const u64 *__restrict__ Bl,
                                                      const i64 *__restrict__ Bse, u64 *__restrict__ Cl,
@@ -1012,6 +1075,13 @@ int axpy_cplx (i64 n, const u64 *al, const i64 *ase, const u64 *xl, const i64 *x
 int gemm_perop (i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl, const i64 *Bse, u64 *Cl,
                 i64 *Cse, hipStream_t s)
 {
+  if (NL <= 8 && k > 0)
+    {
+      const i64 tiles = ((m + GT - 1) / GT) * ((n + GT - 1) / GT);
+      const unsigned g = (unsigned) (tiles > 65536 ? 65536 : tiles);
+      hipLaunchKernelGGL (k_gemm_perop_tiled, dim3 (g), dim3 (256), 0, s, m, n, k, Al, Ase, Bl, Bse, Cl, Cse);
+      return launch_status ("gemm_real");
+    }
   i64 tiles = m * ((n + TPB - 1) / TPB);
   unsigned g = (unsigned) (tiles > 65536 ? 65536 : tiles);
   hipLaunchKernelGGL (k_gemm_perop, dim3 (g), dim3 (TPB), 0, s, m, n, k, Al, Ase, Bl, Bse, Cl, Cse);
