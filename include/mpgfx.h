@@ -126,9 +126,10 @@ int mpgfx_gemv_complex (int pb, int64_t m, int64_t k,
                         const uint64_t *x_limbs, const int64_t *x_se,
                         uint64_t *y_limbs, int64_t *y_se, void *stream);
 
-/* Bytes of device workspace mpgfx_gemm_real needs for this shape (0 for the
- * per-operation mode).  No counterpart in the reference (its kernels need no
- * workspace). */
+/* Bytes of device workspace with which mpgfx_gemm_real runs this shape
+ * asynchronously (the worst case over exponent spreads on the MFMA path; 0
+ * for the per-operation mode).  No counterpart in the reference (its kernels
+ * need no workspace). */
 int mpgfx_gemm_workspace_size (int pb, int mode, int64_t m, int64_t n, int64_t k,
                                size_t *bytes);
 
@@ -139,8 +140,16 @@ int mpgfx_gemm_workspace_size (int pb, int mode, int64_t m, int64_t n, int64_t k
  *  mode MPGFX_GEMM_EXACT: C[i,j] = RNDN(sum_k A[i,k]*B[k,j]) with the sum exact
  *    and ONE rounding (the semantics of mpfr_dot: exact products then
  *    mpfr_sum, src/mpfr_cuda/dot.c:47-54); slice products run on INT8 MFMA.
- * `ws`/`ws_bytes`: caller workspace of at least mpgfx_gemm_workspace_size()
- * bytes; with ws == NULL the library allocates it stream-ordered itself. */
+ * `ws`/`ws_bytes`: caller workspace.  With at least mpgfx_gemm_workspace_size()
+ * bytes the exact mode is asynchronous: the digit count and the path are
+ * decided on the device from the exponents, the call neither synchronises
+ * `stream` nor allocates, and it can be captured in a HIP graph; its
+ * data-dependent outcome is then the int64 at the start of `ws` once the
+ * stream has run the call: MPGFX_OK, or MPGFX_EUNSUPPORTED when the exponent
+ * spread needs more than 2^22 bits of accumulator (C is then +0).  With a
+ * smaller workspace, or ws == NULL, it synchronises `stream` once (after the
+ * exponent scan), allocates what the workspace lacks stream-ordered, and
+ * returns that outcome directly. */
 int mpgfx_gemm_real (int pb, int mode, int64_t m, int64_t n, int64_t k,
                      const uint64_t *A_limbs, const int64_t *A_se,
                      const uint64_t *B_limbs, const int64_t *B_se,

@@ -13,9 +13,9 @@ from .gen import random_real, random_complex, random_array, device_random
 from . import batch
 from . import interop
 from .ops import (elementwise, sqrt, rt_elementwise, rt_axpy, rt_gemv, rt_dot, rt_gemm, axpy, gemv, gemm,
-                  gemm_workspace_size, supported_pb)
+                  gemm_workspace_size, exact_status, supported_pb)
 from ._lib import MPGFXError, lib
 
 __all__ = ["sqrt", "rt_elementwise", "rt_axpy", "rt_gemv", "rt_dot", "rt_gemm", "MPArray", "empty", "equal", "nlimbs", "ZERO_EXP", "random_real", "random_complex",
            "random_array", "device_random", "batch", "interop", "elementwise", "axpy", "gemv", "gemm", "gemm_workspace_size",
-           "supported_pb", "MPGFXError", "lib"]
+           "exact_status", "supported_pb", "MPGFXError", "lib"]

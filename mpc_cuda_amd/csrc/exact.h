@@ -46,12 +46,66 @@ constexpr int EXACT_KCHUNK = 4096;      /* int32 flush period in k (G <= 20) */
  * 1024^3 with exponent spreads below 8 needs 234 MB) */
 constexpr size_t EXACT_NOALLOC_BYTES = (size_t) 256 << 20;
 
-inline int exact_digits (int pb, i64 spread) { return (int) ((pb + spread + 2 + 7) / 8); }
-inline i64 pad_to (i64 x, i64 m) { return (x + m - 1) / m * m; }
+__host__ __device__ inline int exact_digits (int pb, i64 spread) { return (int) ((pb + spread + 2 + 7) / 8); }
+__host__ __device__ inline i64 pad_to (i64 x, i64 m) { return (x + m - 1) / m * m; }
 /* carry-save slots per output element for S digits (diagonals up to 2S-2) */
-inline int exact_slots (int S) { return (8 * (2 * S - 1) + 31) / 32 + 2; }
+__host__ __device__ inline int exact_slots (int S) { return (8 * (2 * S - 1) + 31) / 32 + 2; }
 /* slots for the largest padded digit count */
 constexpr int EXACT_SLMAX = (8 * (2 * EXACT_SPAD_MAX - 1) + 31) / 32 + 2;
+
+/* digit groups: G in {9, 11, 13, 15, 17}, nG = ceil(S/G), zero planes past S
+ * up to G*nG <= EXACT_SPAD_MAX.  Cost model: padded MFMA work (G*nG)^2, times
+ * (1 + 4/G) for the staging and flush overhead of a pass (a pass stages 2G
+ * planes per k step for G^2 MFMAs). */
+__host__ __device__ inline void exact_choose_groups (int S, int *G, int *nG)
+{
+  double best = -1;
+  for (int c = 9; c <= 17; c += 2)
+    {
+      const int ng = (S + c - 1) / c;
+      if (ng * c > EXACT_SPAD_MAX) continue;
+      const double cost = (double) (ng * c) * (ng * c) * (1.0 + 4.0 / c);
+      if (best < 0 || cost < best) { best = cost; *G = c; *nG = ng; }
+    }
+}
+
+/* Headroom of the int64 carry-save slots.  Per k term, a digit position D
+ * collects sum_{t+u=D} DA_t DB_u over at most spad digit pairs of magnitude
+ * <= 2^14, and a slot adds 4 digit positions shifted by 0, 8, 16 and 24 bits:
+ * at most spad * 2^14 * (1 + 2^8 + 2^16 + 2^24) per k.  Summed over kp terms
+ * that must stay below 2^62 (the slots are signed, and the rounding kernel's
+ * carry pass adds a carry); longer sums go to the wide-accumulator path. */
+__host__ __device__ inline bool exact_slots_hold (int spad, i64 kp)
+{
+  const double per_k = (double) spad * 16384.0 * 16843009.0;
+  return (double) kp * per_k < 4.611686018427388e18;        /* 2^62 */
+}
+
+/* wide-accumulator path: bits an output's exact sum can span, and the cap
+ * beyond which the call is refused */
+__host__ __device__ inline i64 exact_span_bits (int pb, i64 sA, i64 sB) { return sA + sB + 2 * (i64) pb + 64 + 2; }
+constexpr i64 EXACT_SPAN_MAX = (i64) 1 << 22;
+
+/* The plan of the asynchronous mode, decided on the device from the exponent
+ * scan (k_exact_plan) and read by every later stage, which the host launches
+ * for the worst case: digit planes laid out for EXACT_SPAD_MAX digits, the
+ * MFMA kernel switching on G, the rounding and the wide-accumulator kernels
+ * each returning at once unless the plan takes their path.  It sits at the
+ * start of the caller's workspace; `status` (the first 8 bytes) is the call's
+ * outcome once the stream has passed it: MPGFX_OK, or MPGFX_EUNSUPPORTED when
+ * the spread needs more than EXACT_SPAN_MAX accumulator bits or more
+ * accumulator memory than the workspace holds (C is then +0). */
+struct exact_dplan
+{
+  i64 status;
+  i64 sA, sB;                           /* spread maxima (bits) */
+  i64 nw;                               /* wide accumulator: 32-bit words per output */
+  i64 lanes;                            /* wide accumulator: outputs accumulated at once */
+  int S, G, nG, slots;
+  int fallback;                         /* 1: the wide-accumulator path */
+};
+constexpr size_t EXACT_DPLAN_BYTES = 256;
+static_assert (sizeof (exact_dplan) <= EXACT_DPLAN_BYTES, "plan block");
 
 /* per-launch plan, filled on the host after the scan */
 struct exact_plan

@@ -1,9 +1,17 @@
 /* exact_gemm.hip -- host orchestration and precision-independent kernels of
  * the exact-dot GEMM (scheme in exact.h; per-precision kernels in
- * exact_pb.h).  The exponent scan has to reach the host before the digit count
- * and the MFMA kernel variant are chosen, so this mode synchronises its stream
- * once, after the scan.  Also here: the static scratch of the no-allocation
- * entry point (mpgfx_gemm_exact_noalloc). */
+ * exact_pb.h).  Two modes:
+ *  - asynchronous (a caller workspace of at least exact_workspace_bound()
+ *    bytes): the digit count, group size and path are decided on the device
+ *    from the exponent scan (k_exact_plan) and every later stage is launched
+ *    for the worst case, reading that plan; no synchronisation and no
+ *    allocation, so the call can be captured in a graph;
+ *  - synchronous (smaller or no workspace): the scan's spreads are copied to
+ *    the host, which sizes the workspace for this call (allocating it
+ *    stream-ordered, or refusing with noalloc) and launches exactly the
+ *    stages it needs.
+ * Also here: the static scratch of the no-allocation entry point
+ * (mpgfx_gemm_exact_noalloc). */
 #include "common.h"
 #include "dispatch.h"
 #include "exact.h"
@@ -103,17 +111,18 @@ __device__ __forceinline__ void exact_fold (const v4i (&acc)[exact_cfg<G>::ND], 
   for (int d = 0; d < exact_cfg<G>::ND; d++) c[(O + d) >> 2] += (i64) acc[d][v] << (8 * ((O + d) & 3));
 }
 
+/* the kernel body for group size G; `lds` holds at least exact_cfg<G>::LDS
+ * bytes */
 template <int G>
-__global__ __launch_bounds__ (256, 2) void k_exact_mfma (i64 mp, i64 np, i64 kp, int nG, int slots,
-                                                       const int8_t *__restrict__ DA, const int8_t *__restrict__ DB,
-                                                       i64 *__restrict__ S64)
+__device__ __forceinline__ void exact_mfma_body (int8_t *lds, i64 mp, i64 np, i64 kp, int nG, int slots,
+                                                 const int8_t *__restrict__ DA, const int8_t *__restrict__ DB,
+                                                 i64 *__restrict__ S64)
 {
   typedef exact_cfg<G> C;
   constexpr int ND = C::ND, PS = C::PS, PLANE = C::PLANE;
   constexpr int KCH = EXACT_KCHUNK;
   static_assert ((long long) G * KCH * 16384 < (1ll << 31), "int32 accumulators could overflow");
   static_assert (4 * G % 4 == 0 && G <= 20, "group size");
-  __shared__ __attribute__ ((aligned (16))) int8_t lds[C::LDS];
 
   const int tid = threadIdx.x, l = tid & 63, w = __builtin_amdgcn_readfirstlane (tid >> 6), wr = w >> 1, wc = w & 1;
   const int r0 = l & 15, g = l >> 4;
@@ -222,6 +231,35 @@ __global__ __launch_bounds__ (256, 2) void k_exact_mfma (i64 mp, i64 np, i64 kp,
     }
 }
 
+/* the kernel for one group size (the synchronous mode) */
+template <int G>
+__global__ __launch_bounds__ (256, 2) void k_exact_mfma (i64 mp, i64 np, i64 kp, int nG, int slots,
+                                                       const int8_t *__restrict__ DA, const int8_t *__restrict__ DB,
+                                                       i64 *__restrict__ S64)
+{
+  __shared__ __attribute__ ((aligned (16))) int8_t lds[exact_cfg<G>::LDS];
+  exact_mfma_body<G> (lds, mp, np, kp, nG, slots, DA, DB, S64);
+}
+
+/* the kernel of the asynchronous mode: group size, group count and slots
+ * from the device plan (a wave-uniform switch), nothing on the wide path */
+__global__ __launch_bounds__ (256, 2) void k_exact_mfma_plan (i64 mp, i64 np, i64 kp, const int8_t *__restrict__ DA,
+                                                            const int8_t *__restrict__ DB, i64 *__restrict__ S64,
+                                                            const exact_dplan *__restrict__ pl)
+{
+  __shared__ __attribute__ ((aligned (16))) int8_t lds[exact_cfg<17>::LDS];
+  if (pl->fallback) return;
+  const int G = pl->G, nG = pl->nG, slots = pl->slots;
+  switch (G)
+    {
+    case 9: exact_mfma_body<9> (lds, mp, np, kp, nG, slots, DA, DB, S64); break;
+    case 11: exact_mfma_body<11> (lds, mp, np, kp, nG, slots, DA, DB, S64); break;
+    case 13: exact_mfma_body<13> (lds, mp, np, kp, nG, slots, DA, DB, S64); break;
+    case 15: exact_mfma_body<15> (lds, mp, np, kp, nG, slots, DA, DB, S64); break;
+    default: exact_mfma_body<17> (lds, mp, np, kp, nG, slots, DA, DB, S64); break;
+    }
+}
+
 int exact_mfma_run (const exact_plan &p, const int8_t *DA, const int8_t *DB, i64 *S64, int grid, hipStream_t s)
 {
 #define EXACT_G(GG)                                                                                         \
@@ -240,6 +278,45 @@ int exact_mfma_run (const exact_plan &p, const int8_t *DA, const int8_t *DB, i64
   return MPGFX_EINVAL;
 }
 
+/* The plan of the asynchronous mode (one lane): the digit count from the
+ * spreads, the group size and count, the slots, or the wide-accumulator path
+ * with its words per output and how many outputs the workspace after the
+ * scan (`room` bytes) can accumulate at once; refused (status) when the span
+ * exceeds EXACT_SPAN_MAX or not one accumulator fits. */
+__global__ void k_exact_plan (int pb, i64 kp, const unsigned long long *__restrict__ spreads,
+                              exact_dplan *__restrict__ pl, i64 room)
+{
+  if (threadIdx.x != 0) return;
+  exact_dplan p;
+  p.status = MPGFX_OK;
+  p.sA = (i64) spreads[0];
+  p.sB = (i64) spreads[1];
+  p.nw = 0;
+  p.lanes = 0;
+  p.G = 17;
+  p.nG = 1;
+  p.slots = 0;
+  const i64 smax = p.sA > p.sB ? p.sA : p.sB;
+  p.S = (smax > 100000) ? EXACT_SMAX + 1 : exact_digits (pb, smax);
+  bool fb = p.S > EXACT_SMAX;
+  if (!fb)
+    {
+      exact_choose_groups (p.S, &p.G, &p.nG);
+      fb = !exact_slots_hold (p.G * p.nG, kp);
+    }
+  if (!fb)
+    p.slots = exact_slots (p.G * p.nG);
+  else
+    {
+      const i64 span = exact_span_bits (pb, p.sA, p.sB);
+      p.nw = (span + 31) / 32 + 1;
+      p.lanes = (span > EXACT_SPAN_MAX) ? 0 : room / (p.nw * 8);
+      if (p.lanes < 1) p.status = MPGFX_EUNSUPPORTED;
+    }
+  p.fallback = fb ? 1 : 0;
+  *pl = p;
+}
+
 static int hip_ok (const char *where, hipError_t e)
 {
   if (e != hipSuccess) { set_error (where, e); return MPGFX_EHIP; }
@@ -254,46 +331,19 @@ static i64 exact_grid (i64 tiles)
   return (g + 7) / 8 * 8;
 }
 
-/* digit groups: G in {9, 11, 13, 15, 17}, nG = ceil(S/G), zero planes past S
- * up to G*nG <= EXACT_SPAD_MAX.  Cost model: padded MFMA work (G*nG)^2, times
- * (1 + 4/G) for the staging and flush overhead of a pass (a pass stages 2G
- * planes per k step for G^2 MFMAs). */
-static void choose_groups (int S, int *G, int *nG)
-{
-  const int cand[5] = {9, 11, 13, 15, 17};
-  double best = -1;
-  for (int c : cand)
-    {
-      const int ng = (S + c - 1) / c;
-      if (ng * c > EXACT_SPAD_MAX) continue;
-      const double cost = (double) (ng * c) * (ng * c) * (1.0 + 4.0 / c);
-      if (best < 0 || cost < best) { best = cost; *G = c; *nG = ng; }
-    }
-}
-
-/* Headroom of the int64 carry-save slots.  Per k term, a digit position D
- * collects sum_{t+u=D} DA_t DB_u over at most spad digit pairs of magnitude
- * <= 2^14, and a slot adds 4 digit positions shifted by 0, 8, 16 and 24 bits:
- * at most spad * 2^14 * (1 + 2^8 + 2^16 + 2^24) per k.  Summed over kp terms
- * that must stay below 2^62 (the slots are signed, and the rounding kernel's
- * carry pass adds a carry); longer sums go to the wide-accumulator path. */
-static bool exact_slots_hold (int spad, i64 kp)
-{
-  const double per_k = (double) spad * 16384.0 * 16843009.0;
-  return (double) kp * per_k < 4.611686018427388e18;        /* 2^62 */
-}
-
-/* bytes of workspace for the MFMA path with digit count up to EXACT_SMAX */
+/* bytes of workspace for the MFMA path with digit count up to EXACT_SMAX,
+ * the device plan included: a workspace of this size runs the call in the
+ * asynchronous mode */
 size_t exact_workspace_bound (int pb, i64 m, i64 n, i64 k)
 {
   (void) pb;
   const i64 mp = pad_to (m, 32), np = pad_to (n, 32), kp = pad_to (k > 0 ? k : 1, 64);
   const i64 spad = EXACT_SPAD_MAX;                       /* G*nG */
-  size_t b = 0;
+  size_t b = EXACT_DPLAN_BYTES;
   b += (size_t) (2 * m + 2 * n + 8) * 8;                 /* scan outputs + spread words */
   b = (size_t) pad_to ((i64) b, 256);
-  b += (size_t) (spad * (mp + np) * kp);                 /* digit planes */
-  b = (size_t) pad_to ((i64) b, 256);
+  b += (size_t) pad_to (spad * mp * kp, 256);            /* digit planes of A */
+  b += (size_t) pad_to (spad * np * kp, 256);            /* digit planes of B */
   b += (size_t) mp * np * EXACT_SLMAX * 8;               /* carry-save slots */
   return b;
 }
@@ -314,16 +364,58 @@ static size_t exact_mfma_bytes (i64 mp, i64 np, i64 kp, int spad)
   return b;
 }
 
-/* The workspace is the caller's `ws` when it is large enough for this call;
- * what it lacks is allocated stream-ordered, or, with noalloc, refused
- * (MPGFX_ENOMEM, before any launch for the scan buffer, after the scan for
- * the rest: the digit count depends on the exponents). */
+/* The asynchronous mode on a workspace of exact_workspace_bound() bytes: the
+ * device plan at its start, then the scan outputs, the digit planes of A and
+ * B for EXACT_SPAD_MAX digits, the carry-save slots; the wide-accumulator path
+ * reuses everything after the scan outputs.  Every stage is launched; those
+ * the plan does not take return at once. */
+static int gemm_exact_async (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase,
+                             const u64 *Bl, const i64 *Bse, u64 *Cl, i64 *Cse, void *ws, size_t ws_bytes,
+                             hipStream_t s)
+{
+  const i64 mp = pad_to (m, 32), np = pad_to (n, 32), kp = pad_to (k, 64);
+  const size_t scan_bytes = (size_t) pad_to ((i64) ((2 * m + 2 * n + 8) * 8), 256);
+  char *const base = (char *) ws;
+  exact_dplan *const pl = (exact_dplan *) base;
+  i64 *emaxA = (i64 *) (base + EXACT_DPLAN_BYTES), *eminA = emaxA + m, *emaxB = eminA + m, *eminB = emaxB + n;
+  unsigned long long *spreads = (unsigned long long *) (eminB + n);
+  char *q = base + EXACT_DPLAN_BYTES + scan_bytes;
+  const i64 room = (i64) (ws_bytes - EXACT_DPLAN_BYTES - scan_bytes);
+  int8_t *DA = (int8_t *) q;
+  int8_t *DB = DA + pad_to ((i64) EXACT_SPAD_MAX * mp * kp, 256);
+  i64 *S64 = (i64 *) (DB + pad_to ((i64) EXACT_SPAD_MAX * np * kp, 256));
+  int rc;
+  hipLaunchKernelGGL (k_exact_zero2, dim3 (1), dim3 (64), 0, s, spreads);
+  hipLaunchKernelGGL (k_exact_scan, dim3 ((unsigned) m), dim3 (256), 0, s, m, k, k, (i64) 1, Ase, emaxA, eminA,
+                      spreads);
+  hipLaunchKernelGGL (k_exact_scan, dim3 ((unsigned) n), dim3 (256), 0, s, n, k, (i64) 1, n, Bse, emaxB, eminB,
+                      spreads + 1);
+  hipLaunchKernelGGL (k_exact_plan, dim3 (1), dim3 (64), 0, s, pb, kp, spreads, pl, room);
+  if ((rc = launch_status ("gemm_exact scan"))) return rc;
+  if ((rc = o->exact_digits (m, k, k, 1, Al, Ase, m * k, emaxA, 0, 0, 0, mp, kp, DA, pl, 0, s))) return rc;
+  if ((rc = o->exact_digits (n, k, 1, n, Bl, Bse, k * n, emaxB, 0, 0, 0, np, kp, DB, pl, 1, s))) return rc;
+  hipLaunchKernelGGL (k_exact_mfma_plan, dim3 ((unsigned) exact_grid ((mp / 32) * (np / 32))), dim3 (256), 0, s, mp,
+                      np, kp, DA, DB, S64, pl);
+  if ((rc = launch_status ("gemm_exact mfma"))) return rc;
+  if ((rc = o->exact_round (m, n, np, mp * np, 0, S64, emaxA, emaxB, 0, 0, Cl, Cse, pl, s))) return rc;
+  return o->exact_fallback_dev (m, n, k, Al, Ase, Bl, Bse, eminA, eminB, (i64 *) q, Cl, Cse, pl, s);
+}
+
+/* A caller workspace of exact_workspace_bound() bytes takes the asynchronous
+ * mode (the MFMA kernel's 32-bit plane offsets checked for the largest
+ * group).  Otherwise the synchronous one: the workspace is the caller's `ws`
+ * when it is large enough for this call; what it lacks is allocated
+ * stream-ordered, or, with noalloc, refused (MPGFX_ENOMEM, before any launch
+ * for the scan buffer, after the scan for the rest: the digit count depends
+ * on the exponents). */
 int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl,
                 const i64 *Bse, u64 *Cl, i64 *Cse, void *ws, size_t ws_bytes, hipStream_t s, bool noalloc)
 {
   if (k == 0)                      /* empty sums: +0 everywhere, as the per-op path */
     return o->gemm_perop (m, n, k, Al, Ase, Bl, Bse, Cl, Cse, s);
   const i64 mp = pad_to (m, 32), np = pad_to (n, 32), kp = pad_to (k, 64);
+  if (ws && ws_bytes >= exact_workspace_bound (pb, m, n, k) && (i64) 17 * (mp > np ? mp : np) * kp < ((i64) 1 << 31))
+    return gemm_exact_async (o, pb, m, n, k, Al, Ase, Bl, Bse, Cl, Cse, ws, ws_bytes, s);
   const size_t scan_bytes = (size_t) pad_to ((i64) ((2 * m + 2 * n + 8) * 8), 256);
   char *base = ws ? (char *) ws : nullptr;
   size_t avail = ws ? ws_bytes : 0;
@@ -367,7 +459,7 @@ int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, con
   p.fallback = p.S > EXACT_SMAX;
   if (!p.fallback)
     {
-      choose_groups (p.S, &p.G, &p.nG);
+      exact_choose_groups (p.S, &p.G, &p.nG);
       p.fallback = !exact_slots_hold (p.G * p.nG, kp);
     }
   if (!p.fallback)
@@ -390,15 +482,17 @@ int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, con
       int8_t *DB = (int8_t *) q;
       q += pad_to ((i64) spad * np * kp, 256);
       i64 *S64 = (i64 *) q;
-      if ((rc = o->exact_digits (m, k, k, 1, Al, Ase, m * k, emaxA, p.sA, p.S, spad, mp, kp, DA, s))) return done (rc);
-      if ((rc = o->exact_digits (n, k, 1, n, Bl, Bse, k * n, emaxB, p.sB, p.S, spad, np, kp, DB, s))) return done (rc);
+      if ((rc = o->exact_digits (m, k, k, 1, Al, Ase, m * k, emaxA, p.sA, p.S, spad, mp, kp, DA, nullptr, 0, s)))
+        return done (rc);
+      if ((rc = o->exact_digits (n, k, 1, n, Bl, Bse, k * n, emaxB, p.sB, p.S, spad, np, kp, DB, nullptr, 1, s)))
+        return done (rc);
       if ((rc = exact_mfma_run (p, DA, DB, S64, grid, s))) return done (rc);
-      rc = o->exact_round (m, n, np, mp * np, p.slots, S64, emaxA, emaxB, p.sA, p.sB, Cl, Cse, s);
+      rc = o->exact_round (m, n, np, mp * np, p.slots, S64, emaxA, emaxB, p.sA, p.sB, Cl, Cse, nullptr, s);
       return done (rc);
     }
   /* ---- VALU wide-accumulator fallback, in row chunks ---- */
-  const i64 spanbits = p.sA + p.sB + 2 * (i64) pb + 64 + 2;
-  if (spanbits > ((i64) 1 << 22)) return done (MPGFX_EUNSUPPORTED);
+  const i64 spanbits = exact_span_bits (pb, p.sA, p.sB);
+  if (spanbits > EXACT_SPAN_MAX) return done (MPGFX_EUNSUPPORTED);
   const int nw = (int) ((spanbits + 31) / 32) + 1;
   const size_t row_bytes = (size_t) n * nw * 8;
   const size_t budget = (size_t) 256 << 20;

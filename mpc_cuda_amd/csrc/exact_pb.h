@@ -24,12 +24,22 @@ __device__ __forceinline__ void load_sig (const u64 *__restrict__ l, i64 stride,
 
 /* Digit planes of one operand.  Line L (row of A / column of B), position p
  * (k index); the value sits at index L*ls + p*ps of the MP array (nv values).
- * D[t][L][p] for t < Spad, L < Lpad, p < kp; padding and planes >= S are 0. */
+ * D[t][L][p] for t < Spad, L < Lpad, p < kp; padding and planes >= S are 0.
+ * With a device plan `pl` the spread (sA for which = 0, sB for 1), S and
+ * Spad come from it, and nothing is written when it takes the wide path. */
 __global__ __launch_bounds__ (256) void k_exact_digits (i64 lines, i64 len, i64 ls, i64 ps, const u64 *__restrict__ l,
                                                         const i64 *__restrict__ se, i64 nv,
                                                         const i64 *__restrict__ emax_line, i64 spread, int S,
-                                                        int Spad, i64 Lpad, i64 kp, int8_t *__restrict__ D)
+                                                        int Spad, i64 Lpad, i64 kp, int8_t *__restrict__ D,
+                                                        const exact_dplan *__restrict__ pl, int which)
 {
+  if (pl)
+    {
+      if (pl->fallback) return;
+      spread = which ? pl->sB : pl->sA;
+      S = pl->S;
+      Spad = pl->G * pl->nG;
+    }
   const i64 p = (i64) blockIdx.x * blockDim.x + threadIdx.x;
   const i64 L = blockIdx.y;
   if (p >= kp || L >= Lpad) return;
@@ -148,9 +158,17 @@ __device__ void round_store_int (W *__restrict__ wbuf, int nw, i64 scale, u64 *_
 __global__ __launch_bounds__ (256) void k_exact_round (i64 m, i64 n, i64 np, i64 nE, int slots,
                                                        const i64 *__restrict__ S64, const i64 *__restrict__ emaxA,
                                                        const i64 *__restrict__ emaxB, i64 sA, i64 sB,
-                                                       u64 *__restrict__ Cl, i64 *__restrict__ Cse)
+                                                       u64 *__restrict__ Cl, i64 *__restrict__ Cse,
+                                                       const exact_dplan *__restrict__ pl)
 {
   __shared__ u32 words[256 * EXACT_SLMAX];
+  if (pl)
+    {
+      if (pl->fallback) return;
+      slots = pl->slots;
+      sA = pl->sA;
+      sB = pl->sB;
+    }
   const i64 e = (i64) blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m * n) return;
   const i64 i = e / n, j = e - i * n, el = i * np + j;
@@ -171,20 +189,14 @@ __global__ __launch_bounds__ (256) void k_exact_round (i64 m, i64 n, i64 np, i64
   round_store_int (wb, slots, scale, Cl, Cse, m * n, e);
 }
 
-/* VALU fallback: one lane per output, the exact sum accumulated in `nw`
- * two's complement 32-bit words (one per 64-bit workspace slot) whose LSB
- * weighs 2^(eminA_i + eminB_j - 2pb).  Rows r0 .. r0+rows-1 of C. */
-__global__ __launch_bounds__ (64) void k_exact_fallback (i64 m, i64 n, i64 k, i64 r0, i64 rows,
-                                                         const u64 *__restrict__ Al, const i64 *__restrict__ Ase,
-                                                         const u64 *__restrict__ Bl, const i64 *__restrict__ Bse,
-                                                         const i64 *__restrict__ eminA, const i64 *__restrict__ eminB,
-                                                         int nw, i64 *__restrict__ wsp, u64 *__restrict__ Cl,
-                                                         i64 *__restrict__ Cse)
+/* VALU fallback for one output (i, j): its exact sum accumulated in `nw`
+ * two's complement 32-bit words (one per 64-bit slot of acc) whose LSB weighs
+ * 2^(eminA_i + eminB_j - 2pb), then rounded once. */
+__device__ void exact_fb_output (i64 m, i64 n, i64 k, i64 i, i64 j, const u64 *__restrict__ Al,
+                                 const i64 *__restrict__ Ase, const u64 *__restrict__ Bl, const i64 *__restrict__ Bse,
+                                 const i64 *__restrict__ eminA, const i64 *__restrict__ eminB, int nw,
+                                 i64 *__restrict__ acc, u64 *__restrict__ Cl, i64 *__restrict__ Cse)
 {
-  const i64 e = (i64) blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= rows * n) return;
-  const i64 i = r0 + e / n, j = e % n;
-  i64 *acc = wsp + e * nw;
   for (int s = 0; s < nw; s++) acc[s] = 0;
   const i64 base = (eminA[i] == EXACT_EMPTY || eminB[j] == EXACT_EMPTY) ? 0 : eminA[i] + eminB[j];
   bool any = false;
@@ -231,22 +243,75 @@ __global__ __launch_bounds__ (64) void k_exact_fallback (i64 m, i64 n, i64 k, i6
   round_store_int (acc, nw, base - 2 * (i64) PB, Cl, Cse, m * n, i * n + j);
 }
 
+/* VALU fallback: one lane per output, rows r0 .. r0+rows-1 of C, lane e's
+ * accumulator at wsp + e * nw. */
+__global__ __launch_bounds__ (64) void k_exact_fallback (i64 m, i64 n, i64 k, i64 r0, i64 rows,
+                                                         const u64 *__restrict__ Al, const i64 *__restrict__ Ase,
+                                                         const u64 *__restrict__ Bl, const i64 *__restrict__ Bse,
+                                                         const i64 *__restrict__ eminA, const i64 *__restrict__ eminB,
+                                                         int nw, i64 *__restrict__ wsp, u64 *__restrict__ Cl,
+                                                         i64 *__restrict__ Cse)
+{
+  const i64 e = (i64) blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= rows * n) return;
+  exact_fb_output (m, n, k, r0 + e / n, e % n, Al, Ase, Bl, Bse, eminA, eminB, nw, wsp + e * nw, Cl, Cse);
+}
+
+/* The same on a device plan: returns at once unless the plan takes the wide
+ * path; then the plan's `lanes` lanes (at most the grid's) each own an
+ * accumulator at wsp + lane * nw and take every lanes-th output.  A refused
+ * plan (status set) writes +0 everywhere. */
+__global__ __launch_bounds__ (64) void k_exact_fallback_dev (i64 m, i64 n, i64 k, const u64 *__restrict__ Al,
+                                                             const i64 *__restrict__ Ase, const u64 *__restrict__ Bl,
+                                                             const i64 *__restrict__ Bse,
+                                                             const i64 *__restrict__ eminA,
+                                                             const i64 *__restrict__ eminB, i64 *__restrict__ wsp,
+                                                             u64 *__restrict__ Cl, i64 *__restrict__ Cse,
+                                                             const exact_dplan *__restrict__ pl)
+{
+  if (!pl->fallback) return;
+  const i64 tid = (i64) blockIdx.x * blockDim.x + threadIdx.x, nt = (i64) gridDim.x * blockDim.x;
+  if (pl->status != MPGFX_OK)
+    {
+      cu_freal<PB> z;
+      z.set_zero ();
+      for (i64 e = tid; e < m * n; e += nt) st<PB> (z, Cl, Cse, m * n, e);
+      return;
+    }
+  const i64 lanes = pl->lanes < nt ? pl->lanes : nt;
+  if (tid >= lanes) return;
+  const int nw = (int) pl->nw;
+  for (i64 e = tid; e < m * n; e += lanes)
+    exact_fb_output (m, n, k, e / n, e % n, Al, Ase, Bl, Bse, eminA, eminB, nw, wsp + tid * nw, Cl, Cse);
+}
+
 int exact_digits_launch (i64 lines, i64 len, i64 ls, i64 ps, const u64 *l, const i64 *se, i64 nv, const i64 *emax,
-                         i64 spread, int S, int Spad, i64 Lpad, i64 kp, int8_t *D, hipStream_t s)
+                         i64 spread, int S, int Spad, i64 Lpad, i64 kp, int8_t *D, const exact_dplan *pl, int which,
+                         hipStream_t s)
 {
   dim3 grid ((unsigned) ((kp + 255) / 256), (unsigned) Lpad);
   hipLaunchKernelGGL (k_exact_digits, grid, dim3 (256), 0, s, lines, len, ls, ps, l, se, nv, emax, spread, S, Spad,
-                      Lpad, kp, D);
+                      Lpad, kp, D, pl, which);
   return launch_status ("gemm_exact digits");
 }
 
 int exact_round_launch (i64 m, i64 n, i64 np, i64 nE, int slots, const i64 *S64, const i64 *emaxA, const i64 *emaxB,
-                        i64 sA, i64 sB, u64 *Cl, i64 *Cse, hipStream_t s)
+                        i64 sA, i64 sB, u64 *Cl, i64 *Cse, const exact_dplan *pl, hipStream_t s)
 {
   const i64 tot = m * n;
   hipLaunchKernelGGL (k_exact_round, dim3 ((unsigned) ((tot + 255) / 256)), dim3 (256), 0, s, m, n, np, nE, slots, S64,
-                      emaxA, emaxB, sA, sB, Cl, Cse);
+                      emaxA, emaxB, sA, sB, Cl, Cse, pl);
   return launch_status ("gemm_exact round");
+}
+
+int exact_fallback_dev_launch (i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl, const i64 *Bse,
+                               const i64 *eminA, const i64 *eminB, i64 *wsp, u64 *Cl, i64 *Cse, const exact_dplan *pl,
+                               hipStream_t s)
+{
+  const i64 tot = m * n, nb = (tot + 63) / 64;
+  hipLaunchKernelGGL (k_exact_fallback_dev, dim3 ((unsigned) (nb < 4096 ? nb : 4096)), dim3 (64), 0, s, m, n, k, Al,
+                      Ase, Bl, Bse, eminA, eminB, wsp, Cl, Cse, pl);
+  return launch_status ("gemm_exact fallback");
 }
 
 int exact_fallback_launch (i64 m, i64 n, i64 k, i64 r0, i64 rows, const u64 *Al, const i64 *Ase, const u64 *Bl,

@@ -1137,7 +1137,8 @@ int unpack (int cplx, i64 n, const u64 *l, const i64 *se, void *aos, hipStream_t
 #include "batch_pb.h"
 
 const pb_ops ops = { PB, ew_real, ew_cplx, sqrt_real, axpy_real, axpy_cplx, gemm_perop, gemm_cplx, gemv, pack, unpack,
-                     exact_digits_launch, exact_round_launch, exact_fallback_launch, batch_launch };
+                     exact_digits_launch, exact_round_launch, exact_fallback_launch, exact_fallback_dev_launch,
+                     batch_launch };
 
 } // namespace
 

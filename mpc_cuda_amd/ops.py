@@ -180,17 +180,38 @@ def gemm(A, B, m, n, k, mode="perop", out=None, workspace=None):
         return out
     md = {"perop": _lib.GEMM_PEROP, "exact": _lib.GEMM_EXACT}[mode]
     ws_ptr, ws_bytes = None, 0
+    need = gemm_workspace_size(A.pb, mode, m, n, k) if mode == "exact" else 0
     if workspace is None and mode == "exact":
-        workspace = _workspace(A.se.device, gemm_workspace_size(A.pb, mode, m, n, k))
+        workspace = _workspace(A.se.device, need)
     if workspace is not None:
         ws_ptr, ws_bytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
     _lib.check("mpgfx_gemm_real",
                L.mpgfx_gemm_real(A.pb, md, m, n, k, *A.ptrs(), *B.ptrs(), *out.ptrs(),
                                  ws_ptr, ws_bytes, _stream()))
+    if mode == "exact" and k > 0:
+        # asynchronous mode: the outcome lands in the workspace's first word
+        _LAST_WS[str(A.se.device)] = workspace if ws_bytes >= need else None
     return out
 
 
 _WS = {}
+_LAST_WS = {}
+
+
+def exact_status(device="cuda"):
+    """Outcome of the last real exact GEMM gemm() ran on `device`: with a
+    workspace of gemm_workspace_size() bytes (what gemm() passes) the call is
+    asynchronous and its outcome is the first 8 bytes of that workspace: 0, or
+    EUNSUPPORTED when the exponent spread needs more than 2^22 bits of
+    accumulator (C is then +0).  Waits for the stream."""
+    import torch
+    key = str(torch.device(device) if not isinstance(device, torch.device) else device)
+    if key == "cuda":
+        key = "cuda:%d" % torch.cuda.current_device()
+    t = _LAST_WS.get(key)
+    if t is None:
+        return _lib.OK
+    return int(t[:8].view(torch.int64).item())
 
 
 def _workspace(device, nbytes):

@@ -1,0 +1,99 @@
+"""The exact GEMM's asynchronous mode (a caller workspace of
+mpgfx_gemm_workspace_size() bytes: the plan decided on the device, no
+synchronisation, no allocation): captured in a HIP graph and replayed,
+against the synchronous mode (no workspace: the library sizes and allocates
+it after reading the exponent scan) and the oracle, on the MFMA path and the
+wide-accumulator path; and the refusal of spreads beyond 2^22 accumulator
+bits, reported in the workspace's status word (async) or as the return code
+(sync)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import mpc_cuda_amd as M
+from mpc_cuda_amd import _lib
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def sync_mode(pb, A, B, m, n, k, C):
+    """the C ABI with no workspace: the synchronous mode"""
+    import torch
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return _lib.lib().mpgfx_gemm_real(pb, _lib.GEMM_EXACT, m, n, k, *A.ptrs(), *B.ptrs(), *C.ptrs(), None, 0, s)
+
+
+@pytest.mark.parametrize("pb,m,n,k,lo,hi", [
+    (256, 96, 64, 200, -4, 4),          # MFMA path, G = 17
+    (256, 40, 70, 130, -80, 80),        # MFMA path, a smaller group size
+    (128, 20, 30, 50, -3000, 3000),     # wide-accumulator path
+])
+def test_exact_async_matches_sync_and_oracle(gpu, pb, m, n, k, lo, hi):
+    import torch
+    A = M.random_real(pb, m * k, 61 + pb + k, lo, hi)
+    B = M.random_real(pb, k * n, 62 + pb + k, lo, hi)
+    Ad, Bd = A.cuda(), B.cuda()
+    ra = M.gemm(Ad, Bd, m, n, k, "exact")
+    assert M.exact_status() == _lib.OK
+    rs = M.empty(pb, m * n)
+    assert sync_mode(pb, Ad, Bd, m, n, k, rs) == _lib.OK
+    torch.cuda.synchronize()
+    assert M.equal(ra, rs) == 0
+    assert O.mismatches(ra.pair(), O.gemm(pb, m, n, k, A.pair(), B.pair(), "exact"), ignore_zero_sign=False) == 0
+
+
+def test_exact_async_graph_replays(gpu):
+    """Captured once, replayed three times on inputs rewritten between the
+    replays (eager copies and a synchronisation in between): every replay
+    equals the eager call on the same inputs."""
+    import torch
+    pb, m, n, k = 256, 64, 96, 160
+    sets = [(M.device_random(pb, m * k, 71 + 2 * i, -8, 8), M.device_random(pb, k * n, 72 + 2 * i, -8, 8))
+            for i in range(3)]
+    A, B = M.empty(pb, m * k), M.empty(pb, k * n)
+    C = M.empty(pb, m * n)
+    ws = torch.empty(M.gemm_workspace_size(pb, "exact", m, n, k), dtype=torch.uint8, device="cuda")
+
+    def load(i):
+        A.limbs.copy_(sets[i][0].limbs); A.se.copy_(sets[i][0].se)
+        B.limbs.copy_(sets[i][1].limbs); B.se.copy_(sets[i][1].se)
+
+    load(0)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        M.gemm(A, B, m, n, k, "exact", out=C, workspace=ws)       # warm-up outside the capture
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        M.gemm(A, B, m, n, k, "exact", out=C, workspace=ws)
+    torch.cuda.synchronize()
+    for i in (1, 0, 2):
+        load(i)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        want = M.gemm(sets[i][0], sets[i][1], m, n, k, "exact")
+        torch.cuda.synchronize()
+        assert M.equal(C, want) == 0, i
+        assert int(ws[:8].view(torch.int64).item()) == _lib.OK
+
+
+def test_exact_span_beyond_cap_refused(gpu):
+    """Exponent spreads of about 2^21 in both operands need more than 2^22
+    accumulator bits: the synchronous mode returns EUNSUPPORTED, the
+    asynchronous one sets the status word and writes +0."""
+    import torch
+    pb, m, n, k = 64, 4, 3, 8
+    A = M.random_real(pb, m * k, 81, -(1 << 21), 1 << 21).cuda()
+    B = M.random_real(pb, k * n, 82, -(1 << 21), 1 << 21).cuda()
+    C = M.empty(pb, m * n)
+    assert sync_mode(pb, A, B, m, n, k, C) == _lib.EUNSUPPORTED
+    C.se.fill_(12345)
+    M.gemm(A, B, m, n, k, "exact", out=C)
+    assert M.exact_status() == _lib.EUNSUPPORTED
+    h = C.numpy()
+    assert np.all((h.se >> 1) == M.ZERO_EXP) and np.all(h.limbs == 0)
