@@ -21,45 +21,151 @@
 
 namespace mpgfx {
 
-/* per line (row of A or column of B): largest / smallest exponent of its
- * nonzero values, and the largest spread over all lines (atomicMax) */
-__global__ __launch_bounds__ (256) void k_exact_scan (i64 lines, i64 len, i64 ls, i64 ps, const i64 *__restrict__ se,
-                                                      i64 *__restrict__ emax, i64 *__restrict__ emin,
-                                                      unsigned long long *__restrict__ spread_max)
+/* The exponent scan, in three launches that any number of workgroups can
+ * share: k_exact_init sets every line's largest exponent to EXACT_EMPTY and
+ * smallest to INT64_MAX; k_exact_scan folds 256-position pieces of the rows
+ * of A and 64 x 64 tiles of B (64 consecutive columns per read) into them with
+ * atomicMax / atomicMin; exact_scan_finish (one workgroup) marks the lines
+ * without a nonzero value (both EXACT_EMPTY) and writes each operand's
+ * largest spread, spreads[0] for A, spreads[1] for B. */
+__global__ __launch_bounds__ (256) void k_exact_init (i64 m, i64 n, i64 *__restrict__ emaxA, i64 *__restrict__ eminA,
+                                                      i64 *__restrict__ emaxB, i64 *__restrict__ eminB)
+{
+  const i64 t = (i64) blockIdx.x * 256 + threadIdx.x;
+  if (t < m) { emaxA[t] = EXACT_EMPTY; eminA[t] = INT64_MAX; }
+  if (t < n) { emaxB[t] = EXACT_EMPTY; eminB[t] = INT64_MAX; }
+}
+
+static unsigned exact_init_grid (i64 m, i64 n) { return (unsigned) (((m > n ? m : n) + 255) / 256); }
+
+__global__ __launch_bounds__ (256) void k_exact_scan (i64 m, i64 n, i64 k, const i64 *__restrict__ Ase,
+                                                      const i64 *__restrict__ Bse, i64 *__restrict__ emaxA,
+                                                      i64 *__restrict__ eminA, i64 *__restrict__ emaxB,
+                                                      i64 *__restrict__ eminB)
 {
   __shared__ i64 smax[256], smin[256];
-  const i64 L = blockIdx.x;
+  const i64 kA = (k + 255) / 256, nbA = m * kA;
   i64 hi = INT64_MIN, lo = INT64_MAX;
-  for (i64 p = threadIdx.x; p < len; p += blockDim.x)
+  auto take = [&] (i64 w) {
+    const i64 e = w >> 1;
+    if (e != MPGFX_ZERO_EXP)
+      {
+        hi = e > hi ? e : hi;
+        lo = e < lo ? e : lo;
+      }
+  };
+  auto fold = [&] (i64 *emax, i64 *emin, i64 L) {
+    if (hi == INT64_MIN) return;
+    atomicMax ((long long *) (emax + L), (long long) hi);
+    atomicMin ((long long *) (emin + L), (long long) lo);
+  };
+  if ((i64) blockIdx.x < nbA)
     {
-      i64 e = se[L * ls + p * ps] >> 1;
-      if (e != MPGFX_ZERO_EXP)
+      /* positions [c*256, c*256 + 256) of row L */
+      const i64 L = blockIdx.x / kA, p = (blockIdx.x % kA) * 256 + threadIdx.x;
+      if (p < k) take (Ase[L * k + p]);
+      for (int o = 32; o >= 1; o >>= 1)
         {
-          hi = e > hi ? e : hi;
-          lo = e < lo ? e : lo;
+          const i64 h2 = __shfl_xor (hi, o), l2 = __shfl_xor (lo, o);
+          hi = h2 > hi ? h2 : hi;
+          lo = l2 < lo ? l2 : lo;
+        }
+      if ((threadIdx.x & 63) == 0)
+        {
+          smax[threadIdx.x >> 6] = hi;
+          smin[threadIdx.x >> 6] = lo;
+        }
+      __syncthreads ();
+      if (threadIdx.x == 0)
+        {
+          for (int q = 1; q < 4; q++)
+            {
+              hi = smax[q] > hi ? smax[q] : hi;
+              lo = smin[q] < lo ? smin[q] : lo;
+            }
+          fold (emaxA, eminA, L);
         }
     }
-  smax[threadIdx.x] = hi;
-  smin[threadIdx.x] = lo;
-  __syncthreads ();
-  for (int s = 128; s > 0; s >>= 1)
+  else
     {
-      if ((int) threadIdx.x < s)
+      /* columns [g*64, g*64 + 64) over positions [c*64, c*64 + 64): lane
+       * (jj, ph) takes every fourth position */
+      const i64 kB = (k + 63) / 64, b = (i64) blockIdx.x - nbA, g = b / kB, c = b % kB;
+      const i64 L = g * 64 + (threadIdx.x & 63);
+      const int ph = threadIdx.x >> 6;
+      if (L < n)
+        for (int t = 0; t < 16; t++)
+          {
+            const i64 p = c * 64 + ph + 4 * t;
+            if (p < k) take (Bse[p * n + L]);
+          }
+      smax[threadIdx.x] = hi;
+      smin[threadIdx.x] = lo;
+      __syncthreads ();
+      if (threadIdx.x < 64 && L < n)
         {
-          smax[threadIdx.x] = smax[threadIdx.x + s] > smax[threadIdx.x] ? smax[threadIdx.x + s] : smax[threadIdx.x];
-          smin[threadIdx.x] = smin[threadIdx.x + s] < smin[threadIdx.x] ? smin[threadIdx.x + s] : smin[threadIdx.x];
+          for (int q = 1; q < 4; q++)
+            {
+              const i64 h2 = smax[threadIdx.x + 64 * q], l2 = smin[threadIdx.x + 64 * q];
+              hi = h2 > hi ? h2 : hi;
+              lo = l2 < lo ? l2 : lo;
+            }
+          fold (emaxB, eminB, L);
+        }
+    }
+}
+
+static unsigned exact_scan_grid (i64 m, i64 n, i64 k)
+{
+  return (unsigned) (m * ((k + 255) / 256) + ((n + 63) / 64) * ((k + 63) / 64));
+}
+
+/* one workgroup of 256 lanes; `red` is 512 words of LDS */
+__device__ void exact_scan_finish (i64 m, i64 n, i64 *__restrict__ emaxA, i64 *__restrict__ eminA,
+                                   i64 *__restrict__ emaxB, i64 *__restrict__ eminB,
+                                   unsigned long long *__restrict__ spreads, unsigned long long *red)
+{
+  unsigned long long sa = 0, sb = 0;
+  for (i64 L = threadIdx.x; L < m; L += 256)
+    {
+      const i64 hi = emaxA[L];
+      if (hi == EXACT_EMPTY) eminA[L] = EXACT_EMPTY;
+      else sa = (unsigned long long) (hi - eminA[L]) > sa ? (unsigned long long) (hi - eminA[L]) : sa;
+    }
+  for (i64 L = threadIdx.x; L < n; L += 256)
+    {
+      const i64 hi = emaxB[L];
+      if (hi == EXACT_EMPTY) eminB[L] = EXACT_EMPTY;
+      else sb = (unsigned long long) (hi - eminB[L]) > sb ? (unsigned long long) (hi - eminB[L]) : sb;
+    }
+  red[threadIdx.x] = sa;
+  red[256 + threadIdx.x] = sb;
+  __syncthreads ();
+  for (int st = 128; st > 0; st >>= 1)
+    {
+      if ((int) threadIdx.x < st)
+        {
+          red[threadIdx.x] = red[threadIdx.x + st] > red[threadIdx.x] ? red[threadIdx.x + st] : red[threadIdx.x];
+          red[256 + threadIdx.x] = red[256 + threadIdx.x + st] > red[256 + threadIdx.x] ? red[256 + threadIdx.x + st]
+                                                                                        : red[256 + threadIdx.x];
         }
       __syncthreads ();
     }
   if (threadIdx.x == 0)
     {
-      hi = smax[0];
-      lo = smin[0];
-      bool empty = (hi == INT64_MIN);
-      emax[L] = empty ? EXACT_EMPTY : hi;
-      emin[L] = empty ? EXACT_EMPTY : lo;
-      if (!empty) atomicMax (spread_max, (unsigned long long) (hi - lo));
+      spreads[0] = red[0];
+      spreads[1] = red[256];
     }
+}
+
+/* the synchronous mode's last scan step (the host reads the spreads) */
+__global__ __launch_bounds__ (256) void k_exact_scan_finish (i64 m, i64 n, i64 *__restrict__ emaxA,
+                                                             i64 *__restrict__ eminA, i64 *__restrict__ emaxB,
+                                                             i64 *__restrict__ eminB,
+                                                             unsigned long long *__restrict__ spreads)
+{
+  __shared__ unsigned long long red[512];
+  exact_scan_finish (m, n, emaxA, eminA, emaxB, eminB, spreads, red);
 }
 
 
@@ -278,14 +384,19 @@ int exact_mfma_run (const exact_plan &p, const int8_t *DA, const int8_t *DB, i64
   return MPGFX_EINVAL;
 }
 
-/* The plan of the asynchronous mode (one lane): the digit count from the
- * spreads, the group size and count, the slots, or the wide-accumulator path
- * with its words per output and how many outputs the workspace after the
- * scan (`room` bytes) can accumulate at once; refused (status) when the span
- * exceeds EXACT_SPAN_MAX or not one accumulator fits. */
-__global__ void k_exact_plan (int pb, i64 kp, const unsigned long long *__restrict__ spreads,
-                              exact_dplan *__restrict__ pl, i64 room)
+/* The last scan step, then the plan of the asynchronous mode (one lane): the
+ * digit count from the spreads, the group size and count, the slots, or the
+ * wide-accumulator path with its words per output and how many outputs the
+ * workspace after the scan (`room` bytes) can accumulate at once; refused
+ * (status) when the span exceeds EXACT_SPAN_MAX or not one accumulator fits. */
+__global__ __launch_bounds__ (256) void k_exact_plan (int pb, i64 m, i64 n, i64 kp, i64 *__restrict__ emaxA,
+                                                      i64 *__restrict__ eminA, i64 *__restrict__ emaxB,
+                                                      i64 *__restrict__ eminB,
+                                                      unsigned long long *__restrict__ spreads,
+                                                      exact_dplan *__restrict__ pl, i64 room)
 {
+  __shared__ unsigned long long red[512];
+  exact_scan_finish (m, n, emaxA, eminA, emaxB, eminB, spreads, red);
   if (threadIdx.x != 0) return;
   exact_dplan p;
   p.status = MPGFX_OK;
@@ -348,12 +459,6 @@ size_t exact_workspace_bound (int pb, i64 m, i64 n, i64 k)
   return b;
 }
 
-/* two zero words (a kernel rather than a memset, so the call can be captured) */
-__global__ void k_exact_zero2 (unsigned long long *p)
-{
-  if (threadIdx.x < 2) p[threadIdx.x] = 0ull;
-}
-
 /* bytes the MFMA path needs after the scan buffer: digit planes of A and B,
  * then the carry-save slots */
 static size_t exact_mfma_bytes (i64 mp, i64 np, i64 kp, int spad)
@@ -385,12 +490,11 @@ static int gemm_exact_async (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const
   int8_t *DB = DA + pad_to ((i64) EXACT_SPAD_MAX * mp * kp, 256);
   i64 *S64 = (i64 *) (DB + pad_to ((i64) EXACT_SPAD_MAX * np * kp, 256));
   int rc;
-  hipLaunchKernelGGL (k_exact_zero2, dim3 (1), dim3 (64), 0, s, spreads);
-  hipLaunchKernelGGL (k_exact_scan, dim3 ((unsigned) m), dim3 (256), 0, s, m, k, k, (i64) 1, Ase, emaxA, eminA,
-                      spreads);
-  hipLaunchKernelGGL (k_exact_scan, dim3 ((unsigned) n), dim3 (256), 0, s, n, k, (i64) 1, n, Bse, emaxB, eminB,
-                      spreads + 1);
-  hipLaunchKernelGGL (k_exact_plan, dim3 (1), dim3 (64), 0, s, pb, kp, spreads, pl, room);
+  hipLaunchKernelGGL (k_exact_init, dim3 (exact_init_grid (m, n)), dim3 (256), 0, s, m, n, emaxA, eminA, emaxB, eminB);
+  hipLaunchKernelGGL (k_exact_scan, dim3 (exact_scan_grid (m, n, k)), dim3 (256), 0, s, m, n, k, Ase, Bse, emaxA,
+                      eminA, emaxB, eminB);
+  hipLaunchKernelGGL (k_exact_plan, dim3 (1), dim3 (256), 0, s, pb, m, n, kp, emaxA, eminA, emaxB, eminB, spreads, pl,
+                      room);
   if ((rc = launch_status ("gemm_exact scan"))) return rc;
   if ((rc = o->exact_digits (m, k, k, 1, Al, Ase, m * k, emaxA, 0, 0, 0, mp, kp, DA, pl, 0, s))) return rc;
   if ((rc = o->exact_digits (n, k, 1, n, Bl, Bse, k * n, emaxB, 0, 0, 0, np, kp, DB, pl, 1, s))) return rc;
@@ -441,11 +545,10 @@ int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, con
     if (owned) (void) hipFreeAsync (owned, s);
     return r;
   };
-  hipLaunchKernelGGL (k_exact_zero2, dim3 (1), dim3 (64), 0, s, spreads);
-  hipLaunchKernelGGL (k_exact_scan, dim3 ((unsigned) m), dim3 (256), 0, s, m, k, k, (i64) 1, Ase, emaxA, eminA,
-                      spreads);
-  hipLaunchKernelGGL (k_exact_scan, dim3 ((unsigned) n), dim3 (256), 0, s, n, k, (i64) 1, n, Bse, emaxB, eminB,
-                      spreads + 1);
+  hipLaunchKernelGGL (k_exact_init, dim3 (exact_init_grid (m, n)), dim3 (256), 0, s, m, n, emaxA, eminA, emaxB, eminB);
+  hipLaunchKernelGGL (k_exact_scan, dim3 (exact_scan_grid (m, n, k)), dim3 (256), 0, s, m, n, k, Ase, Bse, emaxA,
+                      eminA, emaxB, eminB);
+  hipLaunchKernelGGL (k_exact_scan_finish, dim3 (1), dim3 (256), 0, s, m, n, emaxA, eminA, emaxB, eminB, spreads);
   if ((rc = launch_status ("gemm_exact scan"))) return done (rc);
   unsigned long long hs[2] = {0, 0};
   if ((rc = hip_ok ("gemm_exact spread copy", hipMemcpyAsync (hs, spreads, sizeof hs, hipMemcpyDeviceToHost, s))))

@@ -22,38 +22,21 @@ __device__ __forceinline__ void load_sig (const u64 *__restrict__ l, i64 stride,
   for (int q = 0; q < KWD; q++) w[q] = all[q + SBW];
 }
 
-/* Digit planes of one operand.  Line L (row of A / column of B), position p
- * (k index); the value sits at index L*ls + p*ps of the MP array (nv values).
- * D[t][L][p] for t < Spad, L < Lpad, p < kp; padding and planes >= S are 0.
- * With a device plan `pl` the spread (sA for which = 0, sB for 1), S and
- * Spad come from it, and nothing is written when it takes the wide path. */
-__global__ __launch_bounds__ (256) void k_exact_digits (i64 lines, i64 len, i64 ls, i64 ps, const u64 *__restrict__ l,
-                                                        const i64 *__restrict__ se, i64 nv,
-                                                        const i64 *__restrict__ emax_line, i64 spread, int S,
-                                                        int Spad, i64 Lpad, i64 kp, int8_t *__restrict__ D,
-                                                        const exact_dplan *__restrict__ pl, int which)
+/* The S signed digits of one value, put(t, d) for every plane t < Spad
+ * (planes past S, and every plane of a value that is not live, get 0): the
+ * value as an integer aligned to its line's largest exponent em (M' * 2^sh,
+ * sh = spread - (em - e)), negated for a negative sign, in base-256 digits
+ * recoded to [-128, 127] with a carry (the top digit keeps the sign). */
+template <typename Put>
+__device__ __forceinline__ void exact_value_digits (const u64 *__restrict__ l, const i64 *__restrict__ se, i64 nv,
+                                                    i64 idx, bool live, i64 em, i64 spread, int S, int Spad, Put &&put)
 {
-  if (pl)
-    {
-      if (pl->fallback) return;
-      spread = which ? pl->sB : pl->sA;
-      S = pl->S;
-      Spad = pl->G * pl->nG;
-    }
-  const i64 p = (i64) blockIdx.x * blockDim.x + threadIdx.x;
-  const i64 L = blockIdx.y;
-  if (p >= kp || L >= Lpad) return;
-  const i64 plane = Lpad * kp;
-  int8_t *out = D + L * kp + p;
-  bool live = (L < lines) && (p < len);
-  i64 idx = live ? L * ls + p * ps : 0;
-  i64 w = live ? se[idx] : 0;
-  i64 e = w >> 1;
-  i64 em = live ? emax_line[L] : EXACT_EMPTY;
+  const i64 w = live ? se[idx] : 0;
+  const i64 e = w >> 1;
   live = live && (e != MPGFX_ZERO_EXP) && (em != EXACT_EMPTY);
   if (!live)
     {
-      for (int t = 0; t < Spad; t++) out[(i64) t * plane] = 0;
+      for (int t = 0; t < Spad; t++) put (t, 0);
       return;
     }
   u32 M[KWD];
@@ -69,7 +52,7 @@ __global__ __launch_bounds__ (256) void k_exact_digits (i64 lines, i64 len, i64 
     }
   const u32 neg = (u32) (w & 1);
   u32 cn = neg, c = 0;                            /* negation carry, recode carry */
-  for (int t = 0; t < q && t < Spad; t++) out[(i64) t * plane] = 0;
+  for (int t = 0; t < q && t < Spad; t++) put (t, 0);
 #pragma unroll
   for (int b = 0; b < 4 * (KWD + 1); b++)
     {
@@ -81,7 +64,7 @@ __global__ __launch_bounds__ (256) void k_exact_digits (i64 lines, i64 len, i64 
         { d = (int) (y >= 128u ? (int) y - 256 : (int) y) + (int) c; c = 0; }
       else
         { u32 v = y + c; c = v >= 128u; d = (int) v - (c ? 256 : 0); }
-      if (t < S) out[(i64) t * plane] = (int8_t) d;
+      if (t < S) put (t, d);
     }
   for (int t = q + 4 * (KWD + 1); t < S; t++)
     {
@@ -92,9 +75,63 @@ __global__ __launch_bounds__ (256) void k_exact_digits (i64 lines, i64 len, i64 
         { d = (int) (y >= 128u ? (int) y - 256 : (int) y) + (int) c; c = 0; }
       else
         { u32 v = y + c; c = v >= 128u; d = (int) v - (c ? 256 : 0); }
-      out[(i64) t * plane] = (int8_t) d;
+      put (t, d);
     }
-  for (int t = (S > q ? S : q); t < Spad; t++) out[(i64) t * plane] = 0;
+  for (int t = (S > q ? S : q); t < Spad; t++) put (t, 0);
+}
+
+/* Digit planes of one operand, D[t][L][p] for t < Spad, L < Lpad, p < kp
+ * (padding and planes >= S are 0).  Lines are the rows of A (value L*len +
+ * p, positions contiguous; COLS false) or the columns of B (value p*lines +
+ * L, lines contiguous; COLS true).  A workgroup of 1024 lanes takes LT lines
+ * x PT positions, each lane one value, read so that consecutive lanes read
+ * consecutive values (positions for rows, lines for columns); its digits go
+ * to an LDS tile [t][line][position], which is written out as PT-byte runs
+ * of positions per line and plane (four bytes per lane), so neither side is
+ * strided in HBM.  With a device plan `pl` the spread (sA for which = 0, sB
+ * for 1), S and Spad come from it, and nothing is written when it takes the
+ * wide path. */
+template <bool COLS>
+struct exd_tile
+{
+  static constexpr int LT = COLS ? 16 : 4, PT = COLS ? 64 : 256;
+};
+template <bool COLS>
+__global__ __launch_bounds__ (1024) void k_exact_digits (i64 lines, i64 len, const u64 *__restrict__ l,
+                                                         const i64 *__restrict__ se, i64 nv,
+                                                         const i64 *__restrict__ emax_line, i64 spread, int S,
+                                                         int Spad, i64 Lpad, i64 kp, int8_t *__restrict__ D,
+                                                         const exact_dplan *__restrict__ pl, int which)
+{
+  constexpr int LT = exd_tile<COLS>::LT, PT = exd_tile<COLS>::PT, TS = LT * PT;
+  static_assert (TS == 1024, "one value per lane");
+  __shared__ __attribute__ ((aligned (16))) int8_t tile[EXACT_SPAD_MAX * TS];
+  if (pl)
+    {
+      if (pl->fallback) return;
+      spread = which ? pl->sB : pl->sA;
+      S = pl->S;
+      Spad = pl->G * pl->nG;
+    }
+  const int jj = COLS ? threadIdx.x % LT : threadIdx.x / PT, pp = COLS ? threadIdx.x / LT : threadIdx.x % PT;
+  const i64 L0 = (i64) blockIdx.x * LT, p0 = (i64) blockIdx.y * PT;
+  const i64 L = L0 + jj, p = p0 + pp;
+  const bool live = (L < lines) && (p < len);
+  const i64 idx = live ? (COLS ? p * lines + L : L * len + p) : 0;
+  int8_t *const mine = tile + jj * PT + pp;
+  exact_value_digits (l, se, nv, idx, live, live ? emax_line[L] : EXACT_EMPTY, spread, S, Spad,
+                      [&] (int t, int d) { mine[t * TS] = (int8_t) d; });
+  __syncthreads ();
+  /* plane t, line r: PT bytes of positions, four per lane */
+  const i64 plane = Lpad * kp;
+  constexpr int WL = PT / 4;                      /* 32-bit words per line and plane */
+  for (int x = threadIdx.x; x < Spad * LT * WL; x += 1024)
+    {
+      const int t = x / (LT * WL), r = (x / WL) % LT, qd = x % WL;
+      const i64 Lr = L0 + r, pr = p0 + 4 * qd;
+      if (Lr < Lpad && pr < kp)
+        *(u32 *) (D + (i64) t * plane + Lr * kp + pr) = *(const u32 *) (tile + t * TS + r * PT + 4 * qd);
+    }
 }
 
 /* Round the two's complement integer held in the first `nw` 32-bit words of
@@ -289,10 +326,26 @@ int exact_digits_launch (i64 lines, i64 len, i64 ls, i64 ps, const u64 *l, const
                          i64 spread, int S, int Spad, i64 Lpad, i64 kp, int8_t *D, const exact_dplan *pl, int which,
                          hipStream_t s)
 {
-  dim3 grid ((unsigned) ((kp + 255) / 256), (unsigned) Lpad);
-  hipLaunchKernelGGL (k_exact_digits, grid, dim3 (256), 0, s, lines, len, ls, ps, l, se, nv, emax, spread, S, Spad,
-                      Lpad, kp, D, pl, which);
-  return launch_status ("gemm_exact digits");
+  if (ps == 1 && ls == len)
+    {
+      /* lines are rows (the A operand) */
+      typedef exd_tile<false> T;
+      dim3 grid ((unsigned) ((Lpad + T::LT - 1) / T::LT), (unsigned) ((kp + T::PT - 1) / T::PT));
+      hipLaunchKernelGGL ((k_exact_digits<false>), grid, dim3 (1024), 0, s, lines, len, l, se, nv, emax, spread, S,
+                          Spad, Lpad, kp, D, pl, which);
+      return launch_status ("gemm_exact digits");
+    }
+  if (ls == 1 && ps == lines)
+    {
+      /* lines are columns (the B operand) */
+      typedef exd_tile<true> T;
+      dim3 grid ((unsigned) ((Lpad + T::LT - 1) / T::LT), (unsigned) ((kp + T::PT - 1) / T::PT));
+      hipLaunchKernelGGL ((k_exact_digits<true>), grid, dim3 (1024), 0, s, lines, len, l, se, nv, emax, spread, S,
+                          Spad, Lpad, kp, D, pl, which);
+      return launch_status ("gemm_exact digits");
+    }
+  set_error_text ("gemm_exact digits: operand layout neither row- nor column-major");
+  return MPGFX_EINVAL;
 }
 
 int exact_round_launch (i64 m, i64 n, i64 np, i64 nE, int slots, const i64 *S64, const i64 *emaxA, const i64 *emaxB,
