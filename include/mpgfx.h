@@ -166,7 +166,8 @@ int mpgfx_gemm_real (int pb, int mode, int64_t m, int64_t n, int64_t k,
  * sizes in mpgfx_last_error() (the size depends on the exponent spread, so
  * the check follows the exponent scan, which writes only the array); pb 256,
  * m = n = k = 1024 with exponent spreads below 8 needs 234 MB.  Like the exact
- * mode it synchronises `stream` once.  Every call on one GPU uses the same
+ * mode without a full-size workspace, it synchronises `stream` once and
+ * returns the data-dependent outcome directly.  Every call on one GPU uses the same
  * array: calls that overlap on one GPU (other host threads, other streams)
  * race and must be serialised by the caller. */
 int mpgfx_gemm_exact_noalloc (int pb, int64_t m, int64_t n, int64_t k,

@@ -507,18 +507,20 @@ static int gemm_exact_async (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const
 
 /* A caller workspace of exact_workspace_bound() bytes takes the asynchronous
  * mode (the MFMA kernel's 32-bit plane offsets checked for the largest
- * group).  Otherwise the synchronous one: the workspace is the caller's `ws`
- * when it is large enough for this call; what it lacks is allocated
- * stream-ordered, or, with noalloc, refused (MPGFX_ENOMEM, before any launch
- * for the scan buffer, after the scan for the rest: the digit count depends
- * on the exponents). */
+ * group).  Otherwise, and always for the no-allocation entry point (whose
+ * caller has no pointer to a status word), the synchronous one: the workspace
+ * is the caller's `ws` when it is large enough for this call; what it lacks
+ * is allocated stream-ordered, or, with noalloc, refused (MPGFX_ENOMEM,
+ * before any launch for the scan buffer, after the scan for the rest: the
+ * digit count depends on the exponents). */
 int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl,
                 const i64 *Bse, u64 *Cl, i64 *Cse, void *ws, size_t ws_bytes, hipStream_t s, bool noalloc)
 {
   if (k == 0)                      /* empty sums: +0 everywhere, as the per-op path */
     return o->gemm_perop (m, n, k, Al, Ase, Bl, Bse, Cl, Cse, s);
   const i64 mp = pad_to (m, 32), np = pad_to (n, 32), kp = pad_to (k, 64);
-  if (ws && ws_bytes >= exact_workspace_bound (pb, m, n, k) && (i64) 17 * (mp > np ? mp : np) * kp < ((i64) 1 << 31))
+  if (!noalloc && ws && ws_bytes >= exact_workspace_bound (pb, m, n, k) &&
+      (i64) 17 * (mp > np ? mp : np) * kp < ((i64) 1 << 31))
     return gemm_exact_async (o, pb, m, n, k, Al, Ase, Bl, Bse, Cl, Cse, ws, ws_bytes, s);
   const size_t scan_bytes = (size_t) pad_to ((i64) ((2 * m + 2 * n + 8) * 8), 256);
   char *base = ws ? (char *) ws : nullptr;
