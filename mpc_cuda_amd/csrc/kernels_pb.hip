@@ -737,60 +737,88 @@ __global__ __launch_bounds__ (TPB) void k_gemm_perop (i64 m, i64 n, i64 k, const
  * segments of every limb plane) into LDS once, and each lane reads its A
  * value (shared by the 16 lanes of its row) and its B value (shared by the 16
  * lanes of its column) from there: the global reads per step fall from one
- * value per lane to one eighth. */
+ * value per lane to one eighth.  CPLX: both parts of each value are staged,
+ * and the lane folds acc = acc + a b with MPC_RNDNN products and sums
+ * (demos/matmul_mpc.cu:33-52), both components of acc in its registers. */
 constexpr int GT = 16;
+/* one value of an LDS tile: limb planes, then the sign/exponent plane */
+template <int NP>
+__device__ __forceinline__ cu_freal<PB> tile_val (const u64 (&t)[NP][GT * GT], int x)
+{
+  cu_freal<PB> v;
+#pragma unroll
+  for (int q = 0; q < NP - 1; q++) v.m[q] = t[q][x];
+  const i64 w = (i64) t[NP - 1][x];
+  v.sign = (w & 1) ? -1 : 1;
+  v.exp = (long) (w >> 1);
+  return v;
+}
+
+template <int CPLX>
 __global__ __launch_bounds__ (256) void k_gemm_perop_tiled (i64 m, i64 n, i64 k, const u64 *__restrict__ Al,
                                                            const i64 *__restrict__ Ase, const u64 *__restrict__ Bl,
                                                            const i64 *__restrict__ Bse, u64 *__restrict__ Cl,
                                                            i64 *__restrict__ Cse)
 {
   constexpr int NP = cu_freal<PB>::N + 1;                /* limb planes and the sign/exponent plane */
-  __shared__ u64 sa[NP][GT * GT], sb[NP][GT * GT];
-  const int r = threadIdx.x >> 4, c = threadIdx.x & 15;
-  const i64 tm = (m + GT - 1) / GT, tn = (n + GT - 1) / GT;
-  const i64 nA = m * k, nB = k * n, nC = m * n;
-  for (i64 t = blockIdx.x; t < tm * tn; t += gridDim.x)
+  constexpr int NPART = CPLX ? 2 : 1;                    /* real part, imaginary part */
+  /* launched up to 8 limbs (pb 512); wider precisions compile an empty body */
+  constexpr bool ON = cu_freal<PB>::N <= 8;
+  __shared__ u64 sa[NPART][ON ? NP : 1][GT * GT], sb[NPART][ON ? NP : 1][GT * GT];
+  if constexpr (ON)
     {
-      const i64 i0 = (t / tn) * GT, j0 = (t % tn) * GT;
-      const i64 i = i0 + r, j = j0 + c;
-      cu_freal<PB> acc;
-      acc.set_zero ();
-      for (i64 k0 = 0; k0 < k; k0 += GT)
+      const int r = threadIdx.x >> 4, c = threadIdx.x & 15;
+      const i64 tm = (m + GT - 1) / GT, tn = (n + GT - 1) / GT;
+      const i64 nA = m * k, nB = k * n, nC = m * n;
+      for (i64 t = blockIdx.x; t < tm * tn; t += gridDim.x)
         {
-          /* stage: lane (r, c) copies A[i0 + r, k0 + c] and B[k0 + r, j0 + c] */
-          {
-            const bool va = (i < m) && (k0 + c < k), vb = (k0 + r < k) && (j < n);
-            const i64 ia = va ? i * k + k0 + c : 0, ib = vb ? (k0 + r) * n + j : 0;
+          const i64 i0 = (t / tn) * GT, j0 = (t % tn) * GT;
+          const i64 i = i0 + r, j = j0 + c;
+          cu_freal<PB> acc[NPART];
 #pragma unroll
-            for (int q = 0; q < NP - 1; q++)
-              {
-                sa[q][threadIdx.x] = va ? Al[(i64) q * nA + ia] : 0ull;
-                sb[q][threadIdx.x] = vb ? Bl[(i64) q * nB + ib] : 0ull;
-              }
-            sa[NP - 1][threadIdx.x] = va ? (u64) Ase[ia] : 0ull;
-            sb[NP - 1][threadIdx.x] = vb ? (u64) Bse[ib] : 0ull;
-          }
-          __syncthreads ();
-          const int kb = (k - k0 < GT) ? (int) (k - k0) : GT;
-          for (int kk = 0; kk < kb; kk++)
+          for (int h = 0; h < NPART; h++) acc[h].set_zero ();
+          for (i64 k0 = 0; k0 < k; k0 += GT)
             {
-              cu_freal<PB> a, b;
+              /* stage: lane (r, c) copies A[i0 + r, k0 + c] and B[k0 + r, j0 + c] */
+              {
+                const bool va = (i < m) && (k0 + c < k), vb = (k0 + r < k) && (j < n);
+                const i64 ia = va ? i * k + k0 + c : 0, ib = vb ? (k0 + r) * n + j : 0;
 #pragma unroll
-              for (int q = 0; q < NP - 1; q++)
+                for (int h = 0; h < NPART; h++)
+                  {
+#pragma unroll
+                    for (int q = 0; q < NP - 1; q++)
+                      {
+                        sa[h][q][threadIdx.x] = va ? Al[((i64) h * (NP - 1) + q) * nA + ia] : 0ull;
+                        sb[h][q][threadIdx.x] = vb ? Bl[((i64) h * (NP - 1) + q) * nB + ib] : 0ull;
+                      }
+                    sa[h][NP - 1][threadIdx.x] = va ? (u64) Ase[h * nA + ia] : 0ull;
+                    sb[h][NP - 1][threadIdx.x] = vb ? (u64) Bse[h * nB + ib] : 0ull;
+                  }
+              }
+              __syncthreads ();
+              const int kb = (k - k0 < GT) ? (int) (k - k0) : GT;
+              for (int kk = 0; kk < kb; kk++)
                 {
-                  a.m[q] = sa[q][r * GT + kk];
-                  b.m[q] = sb[q][kk * GT + c];
+                  if constexpr (CPLX)
+                    {
+                      const cu_fcomplex<PB> a (tile_val (sa[0], r * GT + kk), tile_val (sa[1], r * GT + kk));
+                      const cu_fcomplex<PB> b (tile_val (sb[0], kk * GT + c), tile_val (sb[1], kk * GT + c));
+                      cu_fcomplex<PB> s (acc[0], acc[NPART - 1]);
+                      s = cu_fp::cu_cadd<PB> (s, cu_fp::cu_cmul<PB> (a, b));
+                      acc[0] = s.re;
+                      acc[NPART - 1] = s.im;
+                    }
+                  else
+                    acc[0] = cu_fp::cu_fadd<PB> (acc[0], cu_fp::cu_fmul<PB> (tile_val (sa[0], r * GT + kk),
+                                                                             tile_val (sb[0], kk * GT + c)));
                 }
-              const i64 wa = (i64) sa[NP - 1][r * GT + kk], wb = (i64) sb[NP - 1][kk * GT + c];
-              a.sign = (wa & 1) ? -1 : 1;
-              a.exp = (long) (wa >> 1);
-              b.sign = (wb & 1) ? -1 : 1;
-              b.exp = (long) (wb >> 1);
-              acc = cu_fp::cu_fadd<PB> (acc, cu_fp::cu_fmul<PB> (a, b));
+              __syncthreads ();
             }
-          __syncthreads ();
+          if (i < m && j < n)
+#pragma unroll
+            for (int h = 0; h < NPART; h++) st<PB> (acc[h], Cl + (i64) h * (NP - 1) * nC, Cse + h * nC, nC, i * n + j);
         }
-      if (i < m && j < n) st<PB> (acc, Cl, Cse, nC, i * n + j);
     }
 }
 
@@ -1079,7 +1107,7 @@ int gemm_perop (i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *B
     {
       const i64 tiles = ((m + GT - 1) / GT) * ((n + GT - 1) / GT);
       const unsigned g = (unsigned) (tiles > 65536 ? 65536 : tiles);
-      hipLaunchKernelGGL (k_gemm_perop_tiled, dim3 (g), dim3 (256), 0, s, m, n, k, Al, Ase, Bl, Bse, Cl, Cse);
+      hipLaunchKernelGGL ((k_gemm_perop_tiled<0>), dim3 (g), dim3 (256), 0, s, m, n, k, Al, Ase, Bl, Bse, Cl, Cse);
       return launch_status ("gemm_real");
     }
   i64 tiles = m * ((n + TPB - 1) / TPB);
@@ -1097,6 +1125,13 @@ int gemm_cplx (i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl
       hipLaunchKernelGGL (k_gemv_cplx_fused, dim3 ((unsigned) ((m + RC - 1) / RC)), dim3 (64), 0, s, m, k, Al, Ase, Bl,
                           Bse, Cl, Cse);
       return launch_status ("gemv_complex");
+    }
+  if (NL <= 8 && k > 0)
+    {
+      const i64 tiles = ((m + GT - 1) / GT) * ((n + GT - 1) / GT);
+      const unsigned g = (unsigned) (tiles > 65536 ? 65536 : tiles);
+      hipLaunchKernelGGL ((k_gemm_perop_tiled<1>), dim3 (g), dim3 (256), 0, s, m, n, k, Al, Ase, Bl, Bse, Cl, Cse);
+      return launch_status ("gemm_complex");
     }
   i64 tiles = m * ((n + TPB - 1) / TPB);
   unsigned g = (unsigned) (tiles > 65536 ? 65536 : tiles);

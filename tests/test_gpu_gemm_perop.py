@@ -56,3 +56,37 @@ def test_gemm_perop_zero_entries_and_cancellation(gpu):
     assert same(r, O.gemm(pb, m, n, k, A2.pair(), B2.pair(), "perop")) == 0
     h = r.numpy()
     assert np.all((h.se[3 * n:4 * n] >> 1) == M.ZERO_EXP)
+
+
+@pytest.mark.parametrize("pb,m,n,k", [
+    (256, 33, 17, 40),     # ragged in every dimension
+    (256, 1, 20, 5),
+    (64, 18, 35, 33),
+    (96, 17, 16, 19),      # sub-limb precision
+    (512, 16, 17, 18),     # widest precision on the tiled kernel
+    (1024, 7, 9, 11),      # untiled kernel
+])
+def test_gemm_complex_perop_ragged_vs_oracle(gpu, pb, m, n, k):
+    """Complex per-op GEMM (acc = acc + A[i,k] B[k,j], MPC_RNDNN products and
+    sums, k ascending; demos/matmul_mpc.cu:33-52), bit for bit against the
+    oracle, zero-sign fields included."""
+    A = M.random_complex(pb, m * k, 51 + pb + m, -6, 6)
+    B = M.random_complex(pb, k * n, 52 + pb + n, -6, 6)
+    r = M.gemm(A.cuda(), B.cuda(), m, n, k, "perop")
+    assert same(r, O.gemm(pb, m, n, k, A.pair(), B.pair(), "cperop")) == 0
+
+
+def test_gemm_complex_perop_256_sampled(gpu):
+    """pb 256, 512 x 384 x 1024 (a large shape, k over 64 staged blocks):
+    rows 0, 255 and 511 recomputed by the oracle, bit for bit."""
+    pb, m, n, k = 256, 512, 384, 1024
+    A = M.random_complex(pb, m * k, 61, -4, 4)
+    B = M.random_complex(pb, k * n, 62, -4, 4)
+    C = M.gemm(A.cuda(), B.cuda(), m, n, k, "perop").numpy()
+    Ap, Bp = A.pair(), B.pair()
+    for r0 in (0, 255, m - 1):
+        ref = O.gemm(pb, m, n, k, Ap, Bp, "cperop", rows=(r0, r0 + 1))
+        sl = slice(r0 * n, (r0 + 1) * n)
+        for part in (0, 1):
+            assert np.array_equal(C.limbs[part][:, sl], ref[0][part][:, sl]), (r0, part)
+            assert np.array_equal(C.se[part][sl], ref[1][part][sl]), (r0, part)
