@@ -188,8 +188,11 @@ int mpgfx_gemm_complex (int pb, int64_t m, int64_t n, int64_t k,
  * ai*br)).  No counterpart in the reference: the exact mode of
  * mpgfx_gemm_real (mpfr_dot semantics, src/mpfr_cuda/dot.c:47-54) carried to
  * the complex matmul of demos/matmul_mpc.cu:33-52.  Runs as two real exact
- * GEMMs over 2k terms on INT8 MFMA; allocates its operand copies and
- * workspace stream-ordered and synchronises the stream (the exponent scans). */
+ * GEMMs over 2k terms on INT8 MFMA in their asynchronous mode, on one
+ * stream-ordered allocation (the assembled operands and a full-size
+ * workspace), and synchronises the stream once at its end to return the
+ * data-dependent outcome (MPGFX_EUNSUPPORTED for a spread beyond 2^22
+ * accumulator bits). */
 int mpgfx_gemm_complex_exact (int pb, int64_t m, int64_t n, int64_t k,
                               const uint64_t *A_limbs, const int64_t *A_se,
                               const uint64_t *B_limbs, const int64_t *B_se,

@@ -505,6 +505,15 @@ static int gemm_exact_async (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const
   return o->exact_fallback_dev (m, n, k, Al, Ase, Bl, Bse, eminA, eminB, (i64 *) q, Cl, Cse, pl, s);
 }
 
+/* whether a call with this workspace runs in the asynchronous mode: the
+ * worst-case workspace, and the MFMA kernel's 32-bit plane offsets for the
+ * largest group */
+static bool exact_takes_async (int pb, i64 m, i64 n, i64 k, size_t ws_bytes)
+{
+  const i64 mp = pad_to (m, 32), np = pad_to (n, 32), kp = pad_to (k, 64);
+  return ws_bytes >= exact_workspace_bound (pb, m, n, k) && (i64) 17 * (mp > np ? mp : np) * kp < ((i64) 1 << 31);
+}
+
 /* A caller workspace of exact_workspace_bound() bytes takes the asynchronous
  * mode (the MFMA kernel's 32-bit plane offsets checked for the largest
  * group).  Otherwise, and always for the no-allocation entry point (whose
@@ -519,8 +528,7 @@ int gemm_exact (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, con
   if (k == 0)                      /* empty sums: +0 everywhere, as the per-op path */
     return o->gemm_perop (m, n, k, Al, Ase, Bl, Bse, Cl, Cse, s);
   const i64 mp = pad_to (m, 32), np = pad_to (n, 32), kp = pad_to (k, 64);
-  if (!noalloc && ws && ws_bytes >= exact_workspace_bound (pb, m, n, k) &&
-      (i64) 17 * (mp > np ? mp : np) * kp < ((i64) 1 << 31))
+  if (!noalloc && ws && exact_takes_async (pb, m, n, k, ws_bytes))
     return gemm_exact_async (o, pb, m, n, k, Al, Ase, Bl, Bse, Cl, Cse, ws, ws_bytes, s);
   const size_t scan_bytes = (size_t) pad_to ((i64) ((2 * m + 2 * n + 8) * 8), 256);
   char *base = ws ? (char *) ws : nullptr;
@@ -654,19 +662,54 @@ int exact_noalloc_scratch (void **out)
   return MPGFX_OK;
 }
 
-/* sign flip of every value of a real MP array (se ^= 1; zeros are skipped by
- * the exact path whatever their sign field) */
-__global__ __launch_bounds__ (256) void k_negate (i64 n, const i64 *__restrict__ src, i64 *__restrict__ dst)
+/* The operands of the complex exact GEMM as real MP arrays over 2k terms, in
+ * one pass: A2 = [Ar | Ai] (m rows of 2k values), Bre = [Br ; -Bi] and Bim =
+ * [Bi ; Br] (2k rows of n values).  NL limb planes, then the sign/exponent
+ * words; the sign flip of -Bi is se ^= 1 (zeros are skipped by the exact path
+ * whatever their sign field). */
+__global__ __launch_bounds__ (256) void k_exact_cplx_operands (int NL, i64 m, i64 n, i64 k,
+                                                               const u64 *__restrict__ Al,
+                                                               const i64 *__restrict__ Ase,
+                                                               const u64 *__restrict__ Bl,
+                                                               const i64 *__restrict__ Bse, u64 *__restrict__ A2l,
+                                                               i64 *__restrict__ A2s, u64 *__restrict__ Brl,
+                                                               i64 *__restrict__ Brs, u64 *__restrict__ Bil,
+                                                               i64 *__restrict__ Bis)
 {
-  for (i64 i = (i64) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64) gridDim.x * blockDim.x)
-    dst[i] = src[i] ^ 1;
+  const i64 nA = m * k, nB = k * n, nA2 = 2 * nA, nB2 = 2 * nB;
+  const i64 stride = (i64) gridDim.x * blockDim.x;
+  for (i64 t = (i64) blockIdx.x * blockDim.x + threadIdx.x; t < nA2; t += stride)
+    {
+      /* A2[i, c]: c < k from Ar[i, c], else Ai[i, c - k] */
+      const i64 i = t / (2 * k), c = t - i * 2 * k, part = c >= k ? 1 : 0, src = i * k + c - part * k;
+      for (int q = 0; q < NL; q++) A2l[(i64) q * nA2 + t] = Al[((i64) part * NL + q) * nA + src];
+      A2s[t] = Ase[part * nA + src];
+    }
+  for (i64 t = (i64) blockIdx.x * blockDim.x + threadIdx.x; t < nB2; t += stride)
+    {
+      /* row r of the stacked B: r < k from Br (Bre) and Bi (Bim), else Bi
+       * negated (Bre) and Br (Bim) */
+      const bool lo = t < nB;
+      const i64 u = lo ? t : t - nB;
+      for (int q = 0; q < NL; q++)
+        {
+          const u64 br = Bl[(i64) q * nB + u], bi = Bl[((i64) NL + q) * nB + u];
+          Brl[(i64) q * nB2 + t] = lo ? br : bi;
+          Bil[(i64) q * nB2 + t] = lo ? bi : br;
+        }
+      const i64 sr = Bse[u], si = Bse[nB + u];
+      Brs[t] = lo ? sr : (si ^ 1);
+      Bis[t] = lo ? si : sr;
+    }
 }
 
 /* Complex exact GEMM: Re C = RNDN(sum_k ar br - ai bi), Im C = RNDN(sum_k
  * ar bi + ai br), each component one exact sum rounded once.  Both are real
  * exact GEMMs over 2k terms, Re = [Ar | Ai] [Br ; -Bi] and Im = [Ar | Ai]
- * [Bi ; Br], so they run on the MFMA path above; the concatenated operands
- * are assembled with stream-ordered device copies. */
+ * [Bi ; Br], so they run on the MFMA path above, in the asynchronous mode on
+ * one stream-ordered allocation (the assembled operands, then a full-size
+ * workspace); the first GEMM's status word is copied aside before the second
+ * reuses the workspace, and the call reads both once at its end. */
 int gemm_exact_complex (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 *Al, const i64 *Ase, const u64 *Bl,
                         const i64 *Bse, u64 *Cl, i64 *Cse, hipStream_t s)
 {
@@ -678,62 +721,39 @@ int gemm_exact_complex (const pb_ops *o, int pb, i64 m, i64 n, i64 k, const u64 
       int rc = o->gemm_perop (m, n, 0, Al, Ase, Bl, Bse, Cl, Cse, s);
       return rc ? rc : o->gemm_perop (m, n, 0, Al, Ase, Bl, Bse, Cil, Cis, s);
     }
-  const i64 nA = m * k, nB = k * n, nA2 = 2 * nA, nB2 = 2 * nB;
-  const size_t bytes = (size_t) (nA2 + 2 * nB2) * (NL + 1) * 8;
+  const i64 nA2 = 2 * m * k, nB2 = 2 * k * n;
+  const size_t opbytes = (size_t) pad_to ((i64) ((nA2 + 2 * nB2) * (NL + 1) * 8 + 16), 256);
+  const size_t wsb = exact_workspace_bound (pb, m, n, 2 * k);
   char *buf = nullptr;
-  if (hip_ok ("gemm_exact complex operands", hipMallocAsync ((void **) &buf, bytes, s))) return MPGFX_ENOMEM;
+  if (hip_ok ("gemm_exact complex workspace", hipMallocAsync ((void **) &buf, opbytes + wsb, s))) return MPGFX_ENOMEM;
   u64 *A2l = (u64 *) buf;
   i64 *A2s = (i64 *) (A2l + (size_t) NL * nA2);
   u64 *Brl = (u64 *) (A2s + nA2);
   i64 *Brs = (i64 *) (Brl + (size_t) NL * nB2);
   u64 *Bil = (u64 *) (Brs + nB2);
   i64 *Bis = (i64 *) (Bil + (size_t) NL * nB2);
-  const u64 *Arl = Al, *Ail = Al + (size_t) NL * nA;
-  const i64 *Ars = Ase, *Ais = Ase + nA;
-  const u64 *Brl0 = Bl, *Bil0 = Bl + (size_t) NL * nB;
-  const i64 *Brs0 = Bse, *Bis0 = Bse + nB;
-  int rc = MPGFX_OK;
-  /* A2 = [Ar | Ai]: each plane is m rows of 2k values */
-  for (int l = 0; l <= NL && !rc; l++)
-    {
-      const bool se = (l == NL);
-      void *dst = se ? (void *) A2s : (void *) (A2l + (size_t) l * nA2);
-      const void *sr = se ? (const void *) Ars : (const void *) (Arl + (size_t) l * nA);
-      const void *si = se ? (const void *) Ais : (const void *) (Ail + (size_t) l * nA);
-      rc = hip_ok ("gemm_exact complex copy", hipMemcpy2DAsync (dst, 2 * k * 8, sr, k * 8, k * 8, m,
-                                                                hipMemcpyDeviceToDevice, s));
-      if (!rc)
-        rc = hip_ok ("gemm_exact complex copy", hipMemcpy2DAsync ((char *) dst + k * 8, 2 * k * 8, si, k * 8, k * 8, m,
-                                                                  hipMemcpyDeviceToDevice, s));
-    }
-  /* Bre = [Br ; -Bi], Bim = [Bi ; Br]: row blocks, contiguous per plane */
-  for (int l = 0; l < NL && !rc; l++)
-    {
-      const size_t pl = (size_t) nB * 8;
-      rc = hip_ok ("gemm_exact complex copy", hipMemcpyAsync (Brl + (size_t) l * nB2, Brl0 + (size_t) l * nB, pl,
-                                                              hipMemcpyDeviceToDevice, s));
-      if (!rc) rc = hip_ok ("gemm_exact complex copy", hipMemcpyAsync (Brl + (size_t) l * nB2 + nB, Bil0 + (size_t) l * nB,
-                                                                       pl, hipMemcpyDeviceToDevice, s));
-      if (!rc) rc = hip_ok ("gemm_exact complex copy", hipMemcpyAsync (Bil + (size_t) l * nB2, Bil0 + (size_t) l * nB, pl,
-                                                                       hipMemcpyDeviceToDevice, s));
-      if (!rc) rc = hip_ok ("gemm_exact complex copy", hipMemcpyAsync (Bil + (size_t) l * nB2 + nB, Brl0 + (size_t) l * nB,
-                                                                       pl, hipMemcpyDeviceToDevice, s));
-    }
-  if (!rc)
-    {
-      rc = hip_ok ("gemm_exact complex copy", hipMemcpyAsync (Brs, Brs0, (size_t) nB * 8, hipMemcpyDeviceToDevice, s));
-      if (!rc) rc = hip_ok ("gemm_exact complex copy", hipMemcpyAsync (Bis, Bis0, (size_t) nB * 8, hipMemcpyDeviceToDevice, s));
-      if (!rc) rc = hip_ok ("gemm_exact complex copy", hipMemcpyAsync (Bis + nB, Brs0, (size_t) nB * 8, hipMemcpyDeviceToDevice, s));
-    }
-  if (!rc)
-    {
-      const i64 g = (nB + 255) / 256;
-      hipLaunchKernelGGL (k_negate, dim3 ((unsigned) (g < 65536 ? g : 65536)), dim3 (256), 0, s, nB, Bis0, Brs + nB);
-      rc = launch_status ("gemm_exact complex negate");
-    }
-  if (!rc) rc = gemm_exact (o, pb, m, n, 2 * k, A2l, A2s, Brl, Brs, Cl, Cse, nullptr, 0, s, false);
-  if (!rc) rc = gemm_exact (o, pb, m, n, 2 * k, A2l, A2s, Bil, Bis, Cil, Cis, nullptr, 0, s, false);
+  i64 *st0 = Bis + nB2;                             /* the first GEMM's status word */
+  char *ws = buf + opbytes;
+  const i64 tot = nA2 > nB2 ? nA2 : nB2, g = (tot + 255) / 256;
+  hipLaunchKernelGGL (k_exact_cplx_operands, dim3 ((unsigned) (g < 65536 ? g : 65536)), dim3 (256), 0, s, NL, m, n,
+                      k, Al, Ase, Bl, Bse, A2l, A2s, Brl, Brs, Bil, Bis);
+  int rc = launch_status ("gemm_exact complex operands");
+  if (!rc) rc = gemm_exact (o, pb, m, n, 2 * k, A2l, A2s, Brl, Brs, Cl, Cse, ws, wsb, s, false);
+  if (!rc) rc = hip_ok ("gemm_exact complex status", hipMemcpyAsync (st0, ws, 8, hipMemcpyDeviceToDevice, s));
+  if (!rc) rc = gemm_exact (o, pb, m, n, 2 * k, A2l, A2s, Bil, Bis, Cil, Cis, ws, wsb, s, false);
+  i64 hst[2] = {MPGFX_OK, MPGFX_OK};
+  const bool async = exact_takes_async (pb, m, n, 2 * k, wsb);   /* else the return codes carry the outcome */
+  if (!rc && async)
+    rc = hip_ok ("gemm_exact complex status", hipMemcpyAsync (&hst[0], st0, 8, hipMemcpyDeviceToHost, s));
+  if (!rc && async)
+    rc = hip_ok ("gemm_exact complex status", hipMemcpyAsync (&hst[1], ws, 8, hipMemcpyDeviceToHost, s));
   (void) hipFreeAsync (buf, s);
+  if (!rc) rc = hip_ok ("gemm_exact complex sync", hipStreamSynchronize (s));
+  if (!rc && (hst[0] != MPGFX_OK || hst[1] != MPGFX_OK))
+    {
+      set_error_text ("gemm_exact complex: an exponent spread needs more than 2^22 accumulator bits");
+      rc = (int) (hst[0] != MPGFX_OK ? hst[0] : hst[1]);
+    }
   return rc;
 }
 

@@ -97,3 +97,14 @@ def test_exact_span_beyond_cap_refused(gpu):
     assert M.exact_status() == _lib.EUNSUPPORTED
     h = C.numpy()
     assert np.all((h.se >> 1) == M.ZERO_EXP) and np.all(h.limbs == 0)
+
+
+def test_complex_exact_span_beyond_cap_refused(gpu):
+    """The complex exact GEMM runs its two real GEMMs in the asynchronous mode
+    and reads their status words at its end: a spread beyond 2^22
+    accumulator bits comes back as EUNSUPPORTED from the call."""
+    pb, m, n, k = 64, 3, 4, 6
+    A = M.random_complex(pb, m * k, 83, -(1 << 21), 1 << 21).cuda()
+    B = M.random_complex(pb, k * n, 84, -(1 << 21), 1 << 21).cuda()
+    with pytest.raises(M.MPGFXError):
+        M.gemm(A, B, m, n, k, "exact")
