@@ -343,9 +343,9 @@ def gemm_configs(with_cpu):
             A = M.device_random(pb, d * d, 501, lo, hi)
             Bm = M.device_random(pb, d * d, 502, lo, hi)
             Ah, Bh = A.pair(), Bm.pair()
-        # steady state: right after the VALU-bound per-op GEMM the first exact
-        # calls run up to 25 % slower while the clocks recover
-        ms_e = _timed(lambda: M.gemm(A, Bm, d, d, d, "exact", out=C), 10, warm=5)
+        # steady state: right after the VALU-bound per-op GEMM the first dozen
+        # exact calls run up to 25 % slower while the clocks recover
+        ms_e = _timed(lambda: M.gemm(A, Bm, d, d, d, "exact", out=C), 10, warm=15)
         ops = 32 * 32 * 2 * d ** 3
         r = {"ms": ms_e, "value": 2 * d ** 3 / (ms_e * 1e-3), "unit": "MP ops/s",
              "roofline": {"bound": "mfma", "achieved": ops / (ms_e * 1e-3) / 1e12, "peak": INT8_PEAK / 1e12,
