@@ -424,8 +424,13 @@ struct rres
  * it lies within that error of a rounding midpoint, or the computed sum is 0.
  * With 3 or 4 guard words below the kept ones (LW = KW + 2 or KW + 3) the
  * test reads the top 64 guard bits.  p > 0 rounds to p bits (a run-time precision, the
- * operands' low bits zero) rather than to KW words. */
-template <int KW, int LW, int ERRB = 0>
+ * operands' low bits zero) rather than to KW words.
+ * RAW: the operands may be unnormalised by one bit (top bit 31 or 30 of the
+ * top word), their exponents those of the words as they stand; they are
+ * ordered by their normalised exponents and top words, and the larger enters
+ * the window one bit lower (its bottom bit in the guard word) so that the
+ * smaller, shifted right by one bit more, never needs a left shift. */
+template <int KW, int LW, int ERRB = 0, bool RAW = false>
 CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
                              int sb, i64 eb, bool zb, const u32 (&Bi)[LW], int p = 0)
 {
@@ -434,7 +439,17 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
   /* order by exponent, then by the top words (zero is the smallest); only
    * equal exponents and equal top words can leave the difference negative,
    * and it is negated below */
-  bool age = zb || (!za && (ea > eb || (ea == eb && Ai[LW - 1] >= Bi[LW - 1])));
+  i64 ean = ea, ebn = eb;
+  u32 ta = Ai[LW - 1], tb = Bi[LW - 1];
+  if constexpr (RAW)
+    {
+      const u32 ha = (Ai[LW - 1] >> 31) ^ 1u, hb = (Bi[LW - 1] >> 31) ^ 1u;
+      ean = ea - (i64) ha;
+      ebn = eb - (i64) hb;
+      ta = fshl (Ai[LW - 1], Ai[LW - 2], ha);
+      tb = fshl (Bi[LW - 1], Bi[LW - 2], hb);
+    }
+  bool age = zb || (!za && (ean > ebn || (ean == ebn && ta >= tb)));
   u32 A[LW], B[LW];
   CUFP_UNROLL
   for (int i = 0; i < LW; i++)
@@ -446,7 +461,7 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
   i64 eA = age ? ea : eb, eB = age ? eb : ea;
   bool zA = age ? za : zb, zB = age ? zb : za;
   const i64 dmax = 32 * (i64) (LW + 1);
-  i64 d = zB ? dmax : (eA - eB);
+  i64 d = zB ? dmax : (eA - eB + (RAW ? 1 : 0));
   d = d > dmax ? dmax : d;
   u32 ws = (u32) (d >> 5), bs = (u32) (d & 31);
 
@@ -491,9 +506,19 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
   u32 xm = sub ? 0xffffffffu : 0u;
   u32 c = (sub && !st) ? 1u : 0u;
   u32 D[L];
-  D[0] = addc (0u, T[0] ^ xm, c);
-  CUFP_UNROLL
-  for (int i = 0; i < LW; i++) D[i + 1] = addc (A[i], T[i + 1] ^ xm, c);
+  if constexpr (RAW)
+    {
+      /* A one bit lower: its bottom bit in the guard word */
+      D[0] = addc (A[0] << 31, T[0] ^ xm, c);
+      CUFP_UNROLL
+      for (int i = 0; i < LW; i++) D[i + 1] = addc (fshr (i + 1 < LW ? A[i + 1] : 0u, A[i], 1u), T[i + 1] ^ xm, c);
+    }
+  else
+    {
+      D[0] = addc (0u, T[0] ^ xm, c);
+      CUFP_UNROLL
+      for (int i = 0; i < LW; i++) D[i + 1] = addc (A[i], T[i + 1] ^ xm, c);
+    }
   D[L - 1] = addc (0u, T[L - 1] ^ xm, c);
   /* A - B < 0: only with equal exponents (otherwise A > B), so nothing was
    * shifted out and the window holds the exact difference in two's
@@ -590,8 +615,10 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
        * |g - 2^(32 GW - 1)| <= that, tested on the top two guard words */
       constexpr int GW = L - 1 - KW;
       static_assert (GW == 3 || GW == 4, "the test reads the top two of 3 or 4 guard words");
-      static_assert (ERRB + 31 - 32 * (GW - 2) >= 0, "error position below the tested guard bits");
-      const int t = ERRB + 32 - (int) cy + (int) lz - 32 * (GW - 2);
+      /* RAW: A sits one bit lower, so E's bits are worth twice as much */
+      constexpr int R = RAW ? 1 : 0;
+      static_assert (ERRB + 31 - R - 32 * (GW - 2) >= 0, "error position below the tested guard bits");
+      const int t = ERRB + 32 - R - (int) cy + (int) lz - 32 * (GW - 2);
       const u64 x = ((u64) E[GW - 1] << 32) | E[GW - 2];
       bool amb = true;
       if (t < 63)
@@ -604,7 +631,7 @@ CUFP_INL rres<KW> add_round (int sa, i64 ea, bool za, const u32 (&Ai)[LW],
   /* p > 0: round to p bits (run-time precision) instead of KW words */
   u32 co = p ? round_top_p<KW, L - 1> (r.k, E, p, st) : round_top<KW, L - 1> (r.k, E, 0u, st);
   r.sign = zres ? 1 : sA;
-  r.exp = eA + (i64) cy - (i64) lz + (i64) co;
+  r.exp = eA + (RAW ? 1 : 0) + (i64) cy - (i64) lz + (i64) co;
   r.zero = zres;
   return r;
 }

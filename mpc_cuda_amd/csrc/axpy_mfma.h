@@ -244,6 +244,20 @@ CUFP_INL void sweep_finish (term<PB> &t, const carry<PB> &st)
   t.exp -= (i64) sh;
 }
 
+/* the top H words of a swept product as they stand (top bit 31 or 30, the
+ * word below dropped: less than one more unit of error), for add_round's RAW
+ * form, which orders and aligns unnormalised terms itself */
+template <int PB>
+CUFP_INL void sweep_finish_raw (term<PB> &t, const carry<PB> &st)
+{
+  constexpr int H = cfg<PB>::H;
+  t.odd = (st.P[H] >> 30) == 0;
+  cu_fp::core::sfor<0, H> ([&] (auto I) {
+    constexpr int i = decltype (I)::value;
+    t.M[i] = st.P[i + 1];
+  });
+}
+
 /* the whole sweep at once */
 template <int PB, typename VF>
 CUFP_INL void sweep (term<PB> &t, VF &&V, const u32 (&aw)[cfg<PB>::KW])

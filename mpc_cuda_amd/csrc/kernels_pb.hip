@@ -455,7 +455,7 @@ __global__ __launch_bounds__ (AXM_TPB) void k_axpy_cplx_mfma (i64 nchunks, i64 n
             axmf::term<P> &t = tm[q];
             const bool zero = aqe == (i64) CU_FREAL_ZERO_EXP || (sew >> 1) == (i64) CU_FREAL_ZERO_EXP;
             t.exp = aqe + (sew >> 1);
-            axmf::sweep_finish<P> (t, cs);
+            axmf::sweep_finish_raw<P> (t, cs);
             t.zero = zero;
             t.sign = zero ? 1 : (((sea[q] ^ sew) & 1) ? -1 : 1);
             t.exp = zero ? 0 : t.exp;
@@ -465,7 +465,7 @@ __global__ __launch_bounds__ (AXM_TPB) void k_axpy_cplx_mfma (i64 nchunks, i64 n
           });
           /* RNDN(p0 - p1) (real part, p = 0) or RNDN(p0 + p1), one rounding */
           tm[1].sign = (p == 0) ? -tm[1].sign : tm[1].sign;
-          const cu_fp::core::rres<KW> r = cu_fp::core::add_round<KW, H, axmf::ERRB> (
+          const cu_fp::core::rres<KW> r = cu_fp::core::add_round<KW, H, axmf::ERRB, true> (
             tm[0].sign, tm[0].exp, tm[0].zero, tm[0].M, tm[1].sign, tm[1].exp, tm[1].zero, tm[1].M);
           const bool amb = r.amb || tm[0].odd || tm[1].odd;
           cu_freal<P> prod;

@@ -156,7 +156,7 @@ long run (int64_t n, const uint64_t *al, const int64_t *ase, const uint64_t *xl,
                 constexpr int TA = 3 * decltype (Gc)::value, TB = TA + 3 < NT ? TA + 3 : NT;
                 sweep_range<PB, 4 * TA, 4 * TB> (cs, [&] (int ww) { return pw[(4 * TA + ww) * 64 + L]; }, aw);
               });
-              sweep_finish<PB> (t, cs);
+              sweep_finish_raw<PB> (t, cs);
               t.zero = zero;
               t.sign = zero ? 1 : aq.sign * ((sew & 1) ? -1 : 1);
               t.exp = zero ? 0 : t.exp;
@@ -170,7 +170,7 @@ long run (int64_t n, const uint64_t *al, const int64_t *ase, const uint64_t *xl,
           const int64_t i = c * 32 + e;
           term<PB> t0 = tm[L][0], t1 = tm[L][1];
           t1.sign = (p == 0) ? -t1.sign : t1.sign;
-          const cu_fp::core::rres<KW> r = cu_fp::core::add_round<KW, H, ERRB> (t0.sign, t0.exp, t0.zero, t0.M, t1.sign,
+          const cu_fp::core::rres<KW> r = cu_fp::core::add_round<KW, H, ERRB, true> (t0.sign, t0.exp, t0.zero, t0.M, t1.sign,
                                                                               t1.exp, t1.zero, t1.M);
           const bool amb = r.amb || t0.odd || t1.odd;
           cu_freal<PB> prod;
