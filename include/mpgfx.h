@@ -146,8 +146,10 @@ int mpgfx_gemm_workspace_size (int pb, int mode, int64_t m, int64_t n, int64_t k
  * `stream` nor allocates, and it can be captured in a HIP graph; its
  * data-dependent outcome is then the int64 at the start of `ws` once the
  * stream has run the call: MPGFX_OK, or MPGFX_EUNSUPPORTED when the exponent
- * spread needs more than 2^22 bits of accumulator (C is then +0).  With a
- * smaller workspace, or ws == NULL, it synchronises `stream` once (after the
+ * spread needs more than 2^22 bits of accumulator (C is then +0).  (This mode
+ * leaves out digit pairs far below the rounding position and recomputes
+ * exactly the elements whose rounding they could change: C is the same bit
+ * for bit.)  With a smaller workspace, or ws == NULL, it synchronises `stream` once (after the
  * exponent scan), allocates what the workspace lacks stream-ordered, and
  * returns that outcome directly. */
 int mpgfx_gemm_real (int pb, int mode, int64_t m, int64_t n, int64_t k,

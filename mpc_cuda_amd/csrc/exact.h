@@ -86,6 +86,43 @@ __host__ __device__ inline bool exact_slots_hold (int spad, i64 kp)
 __host__ __device__ inline i64 exact_span_bits (int pb, i64 sA, i64 sB) { return sA + sB + 2 * (i64) pb + 64 + 2; }
 constexpr i64 EXACT_SPAN_MAX = (i64) 1 << 22;
 
+/* Digit pairs the asynchronous mode leaves out of the MFMA stage: digit t of
+ * an A value times digit u of a B value with t + u < cut.  Digits are at most
+ * 2^7 in magnitude, so per product those pairs add up to less than
+ * 2^14 sum_{d<cut} (d+1) 256^d < cut 2^(15 + 8(cut-1)), and over kp terms the
+ * integer the MFMA stage leaves is within 2^eb (integer units) of the exact
+ * sum, eb = ceil(lg kp) + ceil(lg cut) + 15 + 8(cut-1).  The rounding stage
+ * marks every element whose rounding that error could change (a value within
+ * 2^eb of a rounding midpoint or of zero, or an error not below an eighth of
+ * the last place), and those are recomputed exactly.  The cut is the largest
+ * with eb <= pb + (sA + sB)/2 - 24: for a sum as large as a product of two
+ * values at the middle of their lines' exponent ranges, the error is 24 bits
+ * below the last place, so marked elements are rare.  cut = 0 (eb = -1): no
+ * pair is left out. */
+__host__ __device__ inline int exact_ceil_lg (i64 x)
+{
+  int r = 0;
+  while (((i64) 1 << r) < x) r++;
+  return r;
+}
+__host__ __device__ inline void exact_choose_cut (int pb, i64 sA, i64 sB, i64 kp, int S, int *cut, int *eb)
+{
+  const i64 target = (i64) pb + (sA + sB) / 2 - 24;
+  int c = 0, e = -1;
+  for (int d = 1; d < 2 * S - 1; d++)
+    {
+      const i64 x = exact_ceil_lg (kp) + exact_ceil_lg (d) + 15 + 8 * (i64) (d - 1);
+      if (x > target) break;
+      c = d;
+      e = (int) x;
+    }
+  *cut = c;
+  *eb = e;
+}
+/* Cse word of an element the rounding stage leaves to the exact fix-up (no
+ * value has this sign/exponent word) */
+constexpr i64 EXACT_FIX_MARK = INT64_MIN;
+
 /* The plan of the asynchronous mode, decided on the device from the exponent
  * scan (k_exact_plan) and read by every later stage, which the host launches
  * for the worst case: digit planes laid out for EXACT_SPAD_MAX digits, the
@@ -99,10 +136,11 @@ struct exact_dplan
 {
   i64 status;
   i64 sA, sB;                           /* spread maxima (bits) */
-  i64 nw;                               /* wide accumulator: 32-bit words per output */
-  i64 lanes;                            /* wide accumulator: outputs accumulated at once */
+  i64 nw;                               /* wide accumulator (and the fix-up): 32-bit words per output */
+  i64 lanes;                            /* wide accumulator (and the fix-up): outputs accumulated at once */
   int S, G, nG, slots;
   int fallback;                         /* 1: the wide-accumulator path */
+  int cut, eb;                          /* MFMA path: diagonals left out, error bound (exact_choose_cut) */
 };
 constexpr size_t EXACT_DPLAN_BYTES = 256;
 static_assert (sizeof (exact_dplan) <= EXACT_DPLAN_BYTES, "plan block");

@@ -193,7 +193,10 @@ typedef __attribute__ ((address_space (3))) void *lds_ptr;
  * Diagonal d of a pass sits at digit G*(ga+gb)+d, i.e. slot (G*(ga+gb)+d)/4
  * shifted by 8*((G*(ga+gb)+d) mod 4) bits; each pass adds its diagonals into
  * the element's carry-save int64 slots, slot-major S64[q*nE + i*np + j] (the
- * first pass of a tile stores them and zeroes the rest).
+ * first pass of a tile stores them and zeroes the rest).  With a cut (the
+ * asynchronous mode, exact_choose_cut) the pairs on global diagonals below it
+ * are left out: passes entirely below it are skipped, the others skip the rows
+ * of A fragments (G MFMAs each) entirely below it.
  *
  * Tiles are assigned XCD-aware: workgroup b runs on XCD b % 8, and the
  * workgroups of one XCD take consecutive tiles (rows of tiles), so the A rows
@@ -220,7 +223,7 @@ __device__ __forceinline__ void exact_fold (const v4i (&acc)[exact_cfg<G>::ND], 
 /* the kernel body for group size G; `lds` holds at least exact_cfg<G>::LDS
  * bytes */
 template <int G>
-__device__ __forceinline__ void exact_mfma_body (int8_t *lds, i64 mp, i64 np, i64 kp, int nG, int slots,
+__device__ __forceinline__ void exact_mfma_body (int8_t *lds, i64 mp, i64 np, i64 kp, int nG, int slots, int cut,
                                                  const int8_t *__restrict__ DA, const int8_t *__restrict__ DB,
                                                  i64 *__restrict__ S64)
 {
@@ -251,10 +254,17 @@ __device__ __forceinline__ void exact_mfma_body (int8_t *lds, i64 mp, i64 np, i6
       const i64 tile = it * nb + xslot;
       if (tile >= tiles) break;
       const i64 ti = tile / tn, tj = tile % tn;
+      bool inited = false;
       for (int ga = 0; ga < nG; ga++)
         for (int gb = 0; gb < nG; gb++)
           {
-            const bool first = (ga == 0) && (gb == 0);
+            /* digit pairs with t + u < lim (local digits) are left out; a
+             * pass with none left is skipped, and the first pass run stores
+             * every slot of the element */
+            const int lim = cut - G * (ga + gb);
+            if (lim > 2 * G - 2) continue;
+            const bool first = !inited;
+            inited = true;
             /* buffer resources over this pass's digit groups (G planes each);
              * 32-bit offsets: G * max(mp, np) * kp < 2^31 is checked on the host */
             const __amdgpu_buffer_rsrc_t rsa = __builtin_amdgcn_make_buffer_rsrc (
@@ -292,9 +302,16 @@ __device__ __forceinline__ void exact_mfma_body (int8_t *lds, i64 mp, i64 np, i6
                     for (int t = 0; t < G; t++)
                       {
                         const v4i fn = (t + 1 < G) ? *(const v4i *) (lds + offa + (t + 1) * PLANE) : fa;
+                        /* a row of A fragments whose diagonals all lie
+                         * below the limit is skipped (one wave-uniform
+                         * branch per row; the pairs of the other rows below
+                         * it are computed, which only makes the sum closer) */
+                        if (t + G - 1 >= lim)
+                          {
 #pragma unroll
-                        for (int u = 0; u < G; u++)
-                          acc[t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8 (fa, fb[u], acc[t + u], 0, 0, 0);
+                            for (int u = 0; u < G; u++)
+                              acc[t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8 (fa, fb[u], acc[t + u], 0, 0, 0);
+                          }
                         fa = fn;
                       }
                     __syncthreads ();
@@ -319,7 +336,9 @@ __device__ __forceinline__ void exact_mfma_body (int8_t *lds, i64 mp, i64 np, i6
                     if (first && kc == 0)
                       {
 #pragma unroll
-                        for (int q = 0; q < PS; q++) sl[(i64) q * nEo] = cs[q];
+                        for (int q = 0; q < PS; q++)
+                          if (s0 + q < slots) sl[(i64) q * nEo] = cs[q];
+                        for (int q = 0; q < s0; q++) S64[(i64) q * nEo + el] = 0;
                         for (int q = s0 + PS; q < slots; q++) S64[(i64) q * nEo + el] = 0;
                       }
                     else
@@ -344,7 +363,7 @@ __global__ __launch_bounds__ (256, 2) void k_exact_mfma (i64 mp, i64 np, i64 kp,
                                                        i64 *__restrict__ S64)
 {
   __shared__ __attribute__ ((aligned (16))) int8_t lds[exact_cfg<G>::LDS];
-  exact_mfma_body<G> (lds, mp, np, kp, nG, slots, DA, DB, S64);
+  exact_mfma_body<G> (lds, mp, np, kp, nG, slots, 0, DA, DB, S64);
 }
 
 /* the kernel of the asynchronous mode: group size, group count and slots
@@ -355,14 +374,14 @@ __global__ __launch_bounds__ (256, 2) void k_exact_mfma_plan (i64 mp, i64 np, i6
 {
   __shared__ __attribute__ ((aligned (16))) int8_t lds[exact_cfg<17>::LDS];
   if (pl->fallback) return;
-  const int G = pl->G, nG = pl->nG, slots = pl->slots;
+  const int G = pl->G, nG = pl->nG, slots = pl->slots, cut = pl->cut;
   switch (G)
     {
-    case 9: exact_mfma_body<9> (lds, mp, np, kp, nG, slots, DA, DB, S64); break;
-    case 11: exact_mfma_body<11> (lds, mp, np, kp, nG, slots, DA, DB, S64); break;
-    case 13: exact_mfma_body<13> (lds, mp, np, kp, nG, slots, DA, DB, S64); break;
-    case 15: exact_mfma_body<15> (lds, mp, np, kp, nG, slots, DA, DB, S64); break;
-    default: exact_mfma_body<17> (lds, mp, np, kp, nG, slots, DA, DB, S64); break;
+    case 9: exact_mfma_body<9> (lds, mp, np, kp, nG, slots, cut, DA, DB, S64); break;
+    case 11: exact_mfma_body<11> (lds, mp, np, kp, nG, slots, cut, DA, DB, S64); break;
+    case 13: exact_mfma_body<13> (lds, mp, np, kp, nG, slots, cut, DA, DB, S64); break;
+    case 15: exact_mfma_body<15> (lds, mp, np, kp, nG, slots, cut, DA, DB, S64); break;
+    default: exact_mfma_body<17> (lds, mp, np, kp, nG, slots, cut, DA, DB, S64); break;
     }
 }
 
@@ -407,6 +426,8 @@ __global__ __launch_bounds__ (256) void k_exact_plan (int pb, i64 m, i64 n, i64 
   p.G = 17;
   p.nG = 1;
   p.slots = 0;
+  p.cut = 0;
+  p.eb = -1;
   const i64 smax = p.sA > p.sB ? p.sA : p.sB;
   p.S = (smax > 100000) ? EXACT_SMAX + 1 : exact_digits (pb, smax);
   bool fb = p.S > EXACT_SMAX;
@@ -415,15 +436,18 @@ __global__ __launch_bounds__ (256) void k_exact_plan (int pb, i64 m, i64 n, i64 
       exact_choose_groups (p.S, &p.G, &p.nG);
       fb = !exact_slots_hold (p.G * p.nG, kp);
     }
+  /* accumulator words and lanes of the wide path, also those of the MFMA
+   * path's exact fix-up (which takes the same memory once the slots are read) */
+  const i64 span = exact_span_bits (pb, p.sA, p.sB);
+  p.nw = (span + 31) / 32 + 1;
+  p.lanes = (span > EXACT_SPAN_MAX) ? 0 : room / (p.nw * 8);
   if (!fb)
-    p.slots = exact_slots (p.G * p.nG);
-  else
     {
-      const i64 span = exact_span_bits (pb, p.sA, p.sB);
-      p.nw = (span + 31) / 32 + 1;
-      p.lanes = (span > EXACT_SPAN_MAX) ? 0 : room / (p.nw * 8);
-      if (p.lanes < 1) p.status = MPGFX_EUNSUPPORTED;
+      p.slots = exact_slots (p.G * p.nG);
+      if (p.lanes >= 1) exact_choose_cut (pb, p.sA, p.sB, kp, p.S, &p.cut, &p.eb);
     }
+  else if (p.lanes < 1)
+    p.status = MPGFX_EUNSUPPORTED;
   p.fallback = fb ? 1 : 0;
   *pl = p;
 }

@@ -88,13 +88,15 @@ __device__ __forceinline__ void exact_value_digits (const u64 *__restrict__ l, c
  * consecutive values (positions for rows, lines for columns); its digits go
  * to an LDS tile [t][line][position], which is written out as PT-byte runs
  * of positions per line and plane (four bytes per lane), so neither side is
- * strided in HBM.  With a device plan `pl` the spread (sA for which = 0, sB
- * for 1), S and Spad come from it, and nothing is written when it takes the
- * wide path. */
+ * strided in HBM.  The tile holds TP = 64 planes (64 KB, so two workgroups
+ * share a CU and one's loads overlap the other's stores); a value with more
+ * digit planes is recoded once per 64 of them.  With a device plan `pl` the
+ * spread (sA for which = 0, sB for 1), S and Spad come from it, and nothing is
+ * written when it takes the wide path. */
 template <bool COLS>
 struct exd_tile
 {
-  static constexpr int LT = COLS ? 16 : 4, PT = COLS ? 64 : 256;
+  static constexpr int LT = COLS ? 16 : 4, PT = COLS ? 64 : 256, TP = 64;
 };
 template <bool COLS>
 __global__ __launch_bounds__ (1024) void k_exact_digits (i64 lines, i64 len, const u64 *__restrict__ l,
@@ -103,9 +105,9 @@ __global__ __launch_bounds__ (1024) void k_exact_digits (i64 lines, i64 len, con
                                                          int Spad, i64 Lpad, i64 kp, int8_t *__restrict__ D,
                                                          const exact_dplan *__restrict__ pl, int which)
 {
-  constexpr int LT = exd_tile<COLS>::LT, PT = exd_tile<COLS>::PT, TS = LT * PT;
+  constexpr int LT = exd_tile<COLS>::LT, PT = exd_tile<COLS>::PT, TP = exd_tile<COLS>::TP, TS = LT * PT;
   static_assert (TS == 1024, "one value per lane");
-  __shared__ __attribute__ ((aligned (16))) int8_t tile[EXACT_SPAD_MAX * TS];
+  __shared__ __attribute__ ((aligned (16))) int8_t tile[TP * TS];
   if (pl)
     {
       if (pl->fallback) return;
@@ -118,29 +120,72 @@ __global__ __launch_bounds__ (1024) void k_exact_digits (i64 lines, i64 len, con
   const i64 L = L0 + jj, p = p0 + pp;
   const bool live = (L < lines) && (p < len);
   const i64 idx = live ? (COLS ? p * lines + L : L * len + p) : 0;
+  const i64 em = live ? emax_line[L] : EXACT_EMPTY;
   int8_t *const mine = tile + jj * PT + pp;
-  exact_value_digits (l, se, nv, idx, live, live ? emax_line[L] : EXACT_EMPTY, spread, S, Spad,
-                      [&] (int t, int d) { mine[t * TS] = (int8_t) d; });
-  __syncthreads ();
-  /* plane t, line r: PT bytes of positions, four per lane */
   const i64 plane = Lpad * kp;
   constexpr int WL = PT / 4;                      /* 32-bit words per line and plane */
-  for (int x = threadIdx.x; x < Spad * LT * WL; x += 1024)
+  for (int t0 = 0; t0 < Spad; t0 += TP)
     {
-      const int t = x / (LT * WL), r = (x / WL) % LT, qd = x % WL;
-      const i64 Lr = L0 + r, pr = p0 + 4 * qd;
-      if (Lr < Lpad && pr < kp)
-        *(u32 *) (D + (i64) t * plane + Lr * kp + pr) = *(const u32 *) (tile + t * TS + r * PT + 4 * qd);
+      /* planes t0 .. t0 + TP - 1 of this tile */
+      const int np = Spad - t0 < TP ? Spad - t0 : TP;
+      if (t0 > 0) __syncthreads ();               /* the previous pass's stores have read the tile */
+      exact_value_digits (l, se, nv, idx, live, em, spread, S, Spad, [&] (int t, int d) {
+        const int u = t - t0;
+        if ((unsigned) u < (unsigned) TP) mine[u * TS] = (int8_t) d;
+      });
+      __syncthreads ();
+      /* plane t0 + u, line r: PT bytes of positions, four per lane */
+      for (int x = threadIdx.x; x < np * LT * WL; x += 1024)
+        {
+          const int u = x / (LT * WL), r = (x / WL) % LT, qd = x % WL;
+          const i64 Lr = L0 + r, pr = p0 + 4 * qd;
+          if (Lr < Lpad && pr < kp)
+            *(u32 *) (D + (i64) (t0 + u) * plane + Lr * kp + pr) = *(const u32 *) (tile + u * TS + r * PT + 4 * qd);
+        }
     }
+}
+
+/* Whether an error below 2^eb (integer units) could change the rounding of
+ * the magnitude held in wbuf[0..nw) (top nonzero word h, -1 for zero) to pb
+ * bits: the magnitude is below 2^(eb+2), the error is not below an eighth of
+ * the last place (round bit rb < eb + 2), or bits eb .. rb read 2^(rb-eb) or
+ * 2^(rb-eb) - 1, i.e. the value lies within 2^eb of a rounding midpoint
+ * (tested on the top 64 of those bits when there are more). */
+template <typename W>
+__device__ bool exact_undecided (const W *__restrict__ wbuf, int nw, int h, int eb)
+{
+  if (h < 0) return true;
+  const i64 tb = 32 * (i64) h + 31 - (i64) cu_fp::core::clz32 ((u32) wbuf[h]);
+  const i64 rb = tb - PB;
+  if (rb - eb < 2) return true;
+  auto bits64 = [&] (i64 pos) -> u64 {           /* magnitude bits pos .. pos + 63, pos >= 0 */
+    const i64 w = pos >> 5;
+    const u32 b = (u32) (pos & 31);
+    const u64 x0 = (w < nw) ? (u32) wbuf[w] : 0u, x1 = (w + 1 < nw) ? (u32) wbuf[w + 1] : 0u,
+              x2 = (w + 2 < nw) ? (u32) wbuf[w + 2] : 0u;
+    const u64 lo = x0 | (x1 << 32);
+    return b ? ((lo >> b) | (x2 << (64 - b))) : lo;
+  };
+  const i64 sft = rb - eb;
+  if (sft <= 63)
+    {
+      const u64 mask = (sft == 63) ? ~0ull : ((1ull << (sft + 1)) - 1ull);
+      const u64 q = bits64 (eb) & mask, mid = 1ull << sft;
+      return q == mid || q + 1 == mid;
+    }
+  const u64 q = bits64 (rb - 63), mid = 1ull << 63;
+  return q == mid || q + 1 == mid;
 }
 
 /* Round the two's complement integer held in the first `nw` 32-bit words of
  * wbuf (stored one per 64-bit slot) times 2^scale to pb bits and store it as
  * value i of C.  The words are read at runtime positions from memory, so no
- * register array is indexed dynamically. */
+ * register array is indexed dynamically.  eb >= 0: the integer is within 2^eb
+ * of the exact one, and an element whose rounding that could change gets the
+ * word EXACT_FIX_MARK in Cse instead (the exact fix-up recomputes it). */
 template <typename W>
 __device__ void round_store_int (W *__restrict__ wbuf, int nw, i64 scale, u64 *__restrict__ Cl,
-                                 i64 *__restrict__ Cse, i64 nC, i64 i)
+                                 i64 *__restrict__ Cse, i64 nC, i64 i, int eb = -1)
 {
   /* sign from the top word, then magnitude in place */
   const bool neg = ((u32) wbuf[nw - 1] >> 31) != 0;
@@ -156,6 +201,11 @@ __device__ void round_store_int (W *__restrict__ wbuf, int nw, i64 scale, u64 *_
   int h = -1;
   for (int s = nw - 1; s >= 0; s--)
     if ((u32) wbuf[s] != 0u) { h = s; break; }
+  if (eb >= 0 && exact_undecided (wbuf, nw, h, eb))
+    {
+      Cse[i] = EXACT_FIX_MARK;
+      return;
+    }
   cu_freal<PB> r;
   if (h < 0)
     {
@@ -223,7 +273,7 @@ __global__ __launch_bounds__ (256) void k_exact_round (i64 m, i64 n, i64 np, i64
     }
   const i64 ea = emaxA[i], eb = emaxB[j];
   const i64 scale = (ea == EXACT_EMPTY || eb == EXACT_EMPTY) ? 0 : ea + eb - sA - sB - 2 * (i64) PB;
-  round_store_int (wb, slots, scale, Cl, Cse, m * n, e);
+  round_store_int (wb, slots, scale, Cl, Cse, m * n, e, pl ? pl->eb : -1);
 }
 
 /* VALU fallback for one output (i, j): its exact sum accumulated in `nw`
@@ -294,10 +344,12 @@ __global__ __launch_bounds__ (64) void k_exact_fallback (i64 m, i64 n, i64 k, i6
   exact_fb_output (m, n, k, r0 + e / n, e % n, Al, Ase, Bl, Bse, eminA, eminB, nw, wsp + e * nw, Cl, Cse);
 }
 
-/* The same on a device plan: returns at once unless the plan takes the wide
- * path; then the plan's `lanes` lanes (at most the grid's) each own an
- * accumulator at wsp + lane * nw and take every lanes-th output.  A refused
- * plan (status set) writes +0 everywhere. */
+/* The same on a device plan: on the wide path the plan's `lanes` lanes (at
+ * most the grid's) each own an accumulator at wsp + lane * nw and take every
+ * lanes-th output; a refused plan (status set) writes +0 everywhere.  On the
+ * MFMA path with a cut (the exact fix-up) the same lanes recompute, exactly,
+ * the outputs the rounding stage marked with EXACT_FIX_MARK; without a cut
+ * the kernel returns at once. */
 __global__ __launch_bounds__ (64) void k_exact_fallback_dev (i64 m, i64 n, i64 k, const u64 *__restrict__ Al,
                                                              const i64 *__restrict__ Ase, const u64 *__restrict__ Bl,
                                                              const i64 *__restrict__ Bse,
@@ -306,8 +358,18 @@ __global__ __launch_bounds__ (64) void k_exact_fallback_dev (i64 m, i64 n, i64 k
                                                              u64 *__restrict__ Cl, i64 *__restrict__ Cse,
                                                              const exact_dplan *__restrict__ pl)
 {
-  if (!pl->fallback) return;
+  if (!pl->fallback && pl->cut == 0) return;
   const i64 tid = (i64) blockIdx.x * blockDim.x + threadIdx.x, nt = (i64) gridDim.x * blockDim.x;
+  if (!pl->fallback)
+    {
+      const i64 lanes = pl->lanes < nt ? pl->lanes : nt;
+      if (tid >= lanes) return;
+      const int nw = (int) pl->nw;
+      for (i64 e = tid; e < m * n; e += lanes)
+        if (Cse[e] == EXACT_FIX_MARK)
+          exact_fb_output (m, n, k, e / n, e % n, Al, Ase, Bl, Bse, eminA, eminB, nw, wsp + tid * nw, Cl, Cse);
+      return;
+    }
   if (pl->status != MPGFX_OK)
     {
       cu_freal<PB> z;

@@ -82,6 +82,64 @@ def test_exact_async_graph_replays(gpu):
         assert int(ws[:8].view(torch.int64).item()) == _lib.OK
 
 
+def test_exact_async_fixup_of_undecided_elements(gpu):
+    """The asynchronous mode leaves the digit pairs far below the rounding
+    position out of its MFMA stage (the plan's cut) and recomputes exactly
+    every element whose rounding the left-out part could change.  Planted
+    rows x + y (the rest of the row zero) against columns with 1 in their
+    first two entries: y half an ulp of x (ties to even, x odd and even), just
+    below it (all-ones significand, whose low digits are left out: the
+    truncated sum crosses the midpoint), just above it, minus just below it,
+    and x - x (exact zero); the other columns and rows random.  The cut is
+    read from the plan block at the start of the workspace (int32 at byte
+    60); results against the synchronous mode (no cut) and the oracle."""
+    import torch
+    from mpcases import pack, unpack
+    pb, m, n, k = 256, 20, 8, 64
+    A = M.random_real(pb, m * k, 91, -4, 4)
+    B = M.random_real(pb, k * n, 92, -4, 4)
+    Av, Bv = unpack(A), unpack(B)
+    one = (1, 1, 1 << 255, False)
+    zero = (1, M.ZERO_EXP, 0, True)
+    for j in (1, 4, 6):
+        Bv[0 * n + j] = one
+        Bv[1 * n + j] = one
+    rng = np.random.default_rng(93)
+    top = 1 << 255
+    for r in range(10):
+        i = 2 * r
+        e = int(rng.integers(-4, 5))
+        X = top | (int(rng.integers(1, 1 << 62)) << 2) | (r & 1)       # odd or even
+        case = r % 5
+        if case == 0:
+            y = (1, e - 256, top)                       # half an ulp: a tie
+        elif case == 1:
+            y = (1, e - 257, (1 << 256) - 1)            # just below half an ulp
+        elif case == 2:
+            y = (1, e - 256, top | 1)                   # just above
+        elif case == 3:
+            y = (-1, e - 257, (1 << 256) - 1)           # minus just below
+        else:
+            y = (-1, e, X)                              # x - x
+        Av[i * k + 0] = (1, e, X, False)
+        Av[i * k + 1] = (y[0], y[1], y[2], False)
+        for c in range(2, k):
+            Av[i * k + c] = zero
+    A2, B2 = pack(Av, pb), pack(Bv, pb)
+    Ad, Bd = A2.cuda(), B2.cuda()
+    ws = torch.empty(M.gemm_workspace_size(pb, "exact", m, n, k), dtype=torch.uint8, device="cuda")
+    ra = M.gemm(Ad, Bd, m, n, k, "exact", workspace=ws)
+    torch.cuda.synchronize()
+    assert int(ws[:8].view(torch.int64).item()) == _lib.OK
+    assert int(ws[60:64].view(torch.int32).item()) > 0          # pairs were left out
+    rs = M.empty(pb, m * n)
+    assert sync_mode(pb, Ad, Bd, m, n, k, rs) == _lib.OK
+    torch.cuda.synchronize()
+    assert M.equal(ra, rs) == 0
+    ref = O.gemm(pb, m, n, k, A2.pair(), B2.pair(), "exact")
+    assert O.mismatches(ra.pair(), ref, ignore_zero_sign=False) == 0
+
+
 def test_exact_span_beyond_cap_refused(gpu):
     """Exponent spreads of about 2^21 in both operands need more than 2^22
     accumulator bits: the synchronous mode returns EUNSUPPORTED, the
