@@ -101,23 +101,19 @@ constexpr i64 EXACT_SPAN_MAX = (i64) 1 << 22;
  * pair is left out. */
 __host__ __device__ inline int exact_ceil_lg (i64 x)
 {
-  int r = 0;
-  while (((i64) 1 << r) < x) r++;
-  return r;
+  return x <= 1 ? 0 : 64 - __builtin_clzll ((unsigned long long) (x - 1));
 }
+/* (one lane of the plan kernel: a direct estimate ignoring the lg(cut) term,
+ * then lowered while the bound exceeds the target, at most a few steps) */
 __host__ __device__ inline void exact_choose_cut (int pb, i64 sA, i64 sB, i64 kp, int S, int *cut, int *eb)
 {
-  const i64 target = (i64) pb + (sA + sB) / 2 - 24;
-  int c = 0, e = -1;
-  for (int d = 1; d < 2 * S - 1; d++)
-    {
-      const i64 x = exact_ceil_lg (kp) + exact_ceil_lg (d) + 15 + 8 * (i64) (d - 1);
-      if (x > target) break;
-      c = d;
-      e = (int) x;
-    }
-  *cut = c;
-  *eb = e;
+  const i64 target = (i64) pb + (sA + sB) / 2 - 24, lk = exact_ceil_lg (kp);
+  const i64 room = target - lk - 15;
+  i64 d = room < 0 ? 0 : room / 8 + 1;
+  if (d > 2 * (i64) S - 2) d = 2 * (i64) S - 2;
+  while (d > 0 && lk + exact_ceil_lg (d) + 15 + 8 * (d - 1) > target) d--;
+  *cut = (int) d;
+  *eb = d > 0 ? (int) (lk + exact_ceil_lg (d) + 15 + 8 * (d - 1)) : -1;
 }
 /* Cse word of an element the rounding stage leaves to the exact fix-up (no
  * value has this sign/exponent word) */
