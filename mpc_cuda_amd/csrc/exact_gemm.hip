@@ -236,6 +236,9 @@ __device__ __forceinline__ void exact_mfma_body (int8_t *lds, i64 mp, i64 np, i6
   const int tid = threadIdx.x, l = tid & 63, w = __builtin_amdgcn_readfirstlane (tid >> 6), wr = w >> 1, wc = w & 1;
   const int r0 = l & 15, g = l >> 4;
   const i64 tn = np / 32, tiles = (mp / 32) * tn, nE = mp * np;
+  /* slots below q0 hold only diagonals below the cut: never stored (the
+   * rounding stage reads them as 0; the cut's error bound covers them) */
+  const int q0 = cut / 4;
   const i64 pla = mp * kp, plb = np * kp;
   const i64 nb = gridDim.x;
   const i64 per_xcd = nb / 8, xslot = (i64) (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
@@ -337,18 +340,19 @@ __device__ __forceinline__ void exact_mfma_body (int8_t *lds, i64 mp, i64 np, i6
                       {
 #pragma unroll
                         for (int q = 0; q < PS; q++)
-                          if (s0 + q < slots) sl[(i64) q * nEo] = cs[q];
-                        for (int q = 0; q < s0; q++) S64[(i64) q * nEo + el] = 0;
+                          if (s0 + q >= q0 && s0 + q < slots) sl[(i64) q * nEo] = cs[q];
+                        for (int q = q0; q < s0; q++) S64[(i64) q * nEo + el] = 0;
                         for (int q = s0 + PS; q < slots; q++) S64[(i64) q * nEo + el] = 0;
                       }
                     else
                       {
                         i64 old[PS];
 #pragma unroll
-                        for (int q = 0; q < PS; q++) old[q] = (s0 + q < slots) ? sl[(i64) q * nEo] : 0;
+                        for (int q = 0; q < PS; q++)
+                          old[q] = (s0 + q >= q0 && s0 + q < slots) ? sl[(i64) q * nEo] : 0;
 #pragma unroll
                         for (int q = 0; q < PS; q++)
-                          if (s0 + q < slots) sl[(i64) q * nEo] = old[q] + cs[q];
+                          if (s0 + q >= q0 && s0 + q < slots) sl[(i64) q * nEo] = old[q] + cs[q];
                       }
                   }
               }

@@ -259,9 +259,12 @@ __global__ __launch_bounds__ (256) void k_exact_round (i64 m, i64 n, i64 np, i64
   const i64 e = (i64) blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m * n) return;
   const i64 i = e / n, j = e - i * n, el = i * np + j;
+  /* slots below cut/4 hold only diagonals below the cut: not stored by the
+   * MFMA stage, read as 0 (within the cut's error bound) */
+  const int q0 = pl ? pl->cut / 4 : 0;
   i64 x[EXACT_SLMAX];
 #pragma unroll
-  for (int q = 0; q < EXACT_SLMAX; q++) x[q] = (q < slots) ? S64[(i64) q * nE + el] : 0;
+  for (int q = 0; q < EXACT_SLMAX; q++) x[q] = (q >= q0 && q < slots) ? S64[(i64) q * nE + el] : 0;
   u32 *wb = words + threadIdx.x * EXACT_SLMAX;
   i64 carry = 0;
 #pragma unroll

@@ -13,6 +13,7 @@ import pytest
 
 import mpc_cuda_amd as M
 from mpc_cuda_amd import _lib
+from mpc_cuda_amd.arrays import MPArray
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -82,49 +83,58 @@ def test_exact_async_graph_replays(gpu):
         assert int(ws[:8].view(torch.int64).item()) == _lib.OK
 
 
-def test_exact_async_fixup_of_undecided_elements(gpu):
-    """The asynchronous mode leaves the digit pairs far below the rounding
-    position out of its MFMA stage (the plan's cut) and recomputes exactly
-    every element whose rounding the left-out part could change.  Planted
-    rows x + y (the rest of the row zero) against columns with 1 in their
-    first two entries: y half an ulp of x (ties to even, x odd and even), just
-    below it (all-ones significand, whose low digits are left out: the
-    truncated sum crosses the midpoint), just above it, minus just below it,
-    and x - x (exact zero); the other columns and rows random.  The cut is
-    read from the plan block at the start of the workspace (int32 at byte
-    60); results against the synchronous mode (no cut) and the oracle."""
-    import torch
-    from mpcases import pack, unpack
-    pb, m, n, k = 256, 20, 8, 64
-    A = M.random_real(pb, m * k, 91, -4, 4)
-    B = M.random_real(pb, k * n, 92, -4, 4)
+def _planted_undecided(pb, m, n, k, seed):
+    """Value lists of A (m x k) and B (k x n): the even rows of A are x + y
+    (k indices 0 and 1, the rest of the row zero) with y half an ulp of x (a
+    tie, x odd and even), just below it (all-ones significand, whose low
+    digits fall below the cut: the truncated sum crosses the midpoint), just
+    above it, minus just below it, or -x (exact zero); columns 1, 4 and 6 of
+    B hold 1 in their first two entries; everything else random."""
+    from mpcases import unpack
+    A = M.random_real(pb, m * k, seed, -4, 4)
+    B = M.random_real(pb, k * n, seed + 1, -4, 4)
     Av, Bv = unpack(A), unpack(B)
-    one = (1, 1, 1 << 255, False)
+    one = (1, 1, 1 << (pb - 1), False)
     zero = (1, M.ZERO_EXP, 0, True)
     for j in (1, 4, 6):
         Bv[0 * n + j] = one
         Bv[1 * n + j] = one
-    rng = np.random.default_rng(93)
-    top = 1 << 255
-    for r in range(10):
+    rng = np.random.default_rng(seed + 2)
+    top = 1 << (pb - 1)
+    for r in range(m // 2):
         i = 2 * r
         e = int(rng.integers(-4, 5))
         X = top | (int(rng.integers(1, 1 << 62)) << 2) | (r & 1)       # odd or even
         case = r % 5
         if case == 0:
-            y = (1, e - 256, top)                       # half an ulp: a tie
+            y = (1, e - pb, top)                        # half an ulp: a tie
         elif case == 1:
-            y = (1, e - 257, (1 << 256) - 1)            # just below half an ulp
+            y = (1, e - pb - 1, (1 << pb) - 1)          # just below half an ulp
         elif case == 2:
-            y = (1, e - 256, top | 1)                   # just above
+            y = (1, e - pb, top | 1)                    # just above
         elif case == 3:
-            y = (-1, e - 257, (1 << 256) - 1)           # minus just below
+            y = (-1, e - pb - 1, (1 << pb) - 1)         # minus just below
         else:
             y = (-1, e, X)                              # x - x
         Av[i * k + 0] = (1, e, X, False)
         Av[i * k + 1] = (y[0], y[1], y[2], False)
         for c in range(2, k):
             Av[i * k + c] = zero
+    return Av, Bv
+
+
+def test_exact_async_fixup_of_undecided_elements(gpu):
+    """The asynchronous mode leaves the digit pairs far below the rounding
+    position out of its MFMA stage (the plan's cut) and recomputes exactly
+    every element whose rounding the left-out part could change; planted
+    ties, near-ties and exact zeros (_planted_undecided) among random
+    elements.  The cut is read from the plan block at the start of the
+    workspace (int32 at byte 60); results against the synchronous mode (no
+    cut) and the oracle."""
+    import torch
+    from mpcases import pack
+    pb, m, n, k = 256, 20, 8, 64
+    Av, Bv = _planted_undecided(pb, m, n, k, 91)
     A2, B2 = pack(Av, pb), pack(Bv, pb)
     Ad, Bd = A2.cuda(), B2.cuda()
     ws = torch.empty(M.gemm_workspace_size(pb, "exact", m, n, k), dtype=torch.uint8, device="cuda")
@@ -138,6 +148,32 @@ def test_exact_async_fixup_of_undecided_elements(gpu):
     assert M.equal(ra, rs) == 0
     ref = O.gemm(pb, m, n, k, A2.pair(), B2.pair(), "exact")
     assert O.mismatches(ra.pair(), ref, ignore_zero_sign=False) == 0
+
+
+def test_complex_exact_fixup_of_undecided_elements(gpu):
+    """The complex exact GEMM (two real exact GEMMs over 2k terms in the
+    asynchronous mode) on the planted rows as real parts, with zero
+    imaginary parts on those rows: the ties and near-ties land on Re; against
+    the oracle."""
+    from mpcases import pack, unpack
+    pb, m, n, k = 256, 20, 8, 64
+    Av, Bv = _planted_undecided(pb, m, n, k, 95)
+    Ai = M.random_real(pb, m * k, 97, -4, 4)
+    Bi = M.random_real(pb, k * n, 98, -4, 4)
+    Aiv, Biv = unpack(Ai), unpack(Bi)
+    zero = (1, M.ZERO_EXP, 0, True)
+    for i in range(0, m, 2):
+        for c in range(k):
+            Aiv[i * k + c] = zero
+    Ar, Air = pack(Av, pb), pack(Aiv, pb)
+    Br, Bir = pack(Bv, pb), pack(Biv, pb)
+    A = MPArray(pb, np.ascontiguousarray(np.stack([Ar.limbs, Air.limbs])),
+                np.ascontiguousarray(np.stack([Ar.se, Air.se])), True)
+    B = MPArray(pb, np.ascontiguousarray(np.stack([Br.limbs, Bir.limbs])),
+                np.ascontiguousarray(np.stack([Br.se, Bir.se])), True)
+    r = M.gemm(A.cuda(), B.cuda(), m, n, k, "exact")
+    ref = O.gemm(pb, m, n, k, A.pair(), B.pair(), "cexact")
+    assert O.mismatches(r.pair(), ref, ignore_zero_sign=False) == 0
 
 
 def test_exact_span_beyond_cap_refused(gpu):
