@@ -858,57 +858,76 @@ constexpr int COOP_G = coop_lanes (PB);
 constexpr int COOP_R = 64 / COOP_G;
 
 /* GEMV y_i = fold_k RNDN(acc + RNDN(A[i,k] x[k])), acc from +0, k ascending
- * (tools/linalg_fast.cu:37-51), with the products computed inside the
- * cooperative fold (coop_dpp.h): a wavefront folds R = 64/G rows.  Per block of G steps
- * every lane computes one product, RNDN(A[i_r, s0+j] x[s0+j]) for its row r
- * and step s0 + j, and writes it as a record (N limbs, sign/exponent word)
- * to LDS; the G steps of the block then fold the records of their row, read
- * from LDS at the start of the block.  The next block's operands are loaded
- * from HBM while the current block folds.  No scratch memory. */
+ * (tools/linalg_fast.cu:37-51), with the products computed beside the
+ * cooperative fold (coop_dpp.h).  A fold wavefront holds one chain per row
+ * (one folder per SIMD at 4096 rows), so the products go to a second
+ * wavefront that would otherwise leave the SIMD idle: a workgroup is two
+ * wavefronts over R = 64/G rows: the producer (wavefront 1) computes, per block of G steps, one
+ * product per lane, RNDN(A[i_r, s0+j] x[s0+j]) for row r and step s0 + j,
+ * and writes it as a record (N limbs, sign/exponent word) to one of two LDS
+ * buffers; the folder (wavefront 0) reads block b's records of its row from
+ * the other buffer and folds its G dependent steps while the producer
+ * computes block b + 1.  One barrier per block in each wavefront, the same
+ * count in both (the producer's last one after its loop, the folder's first
+ * one before its loop).  No scratch memory. */
 constexpr int FG_G = mpgfx::coopd::lanes_for (PB);
 constexpr int FG_R = 64 / FG_G;
 
-__global__ __launch_bounds__ (64) void k_gemv_fused (i64 m, i64 k, const u64 *__restrict__ Al,
-                                                     const i64 *__restrict__ Ase, const u64 *__restrict__ xl,
-                                                     const i64 *__restrict__ xse, u64 *__restrict__ yl,
-                                                     i64 *__restrict__ yse)
+__global__ __launch_bounds__ (128) void k_gemv_fused (i64 m, i64 k, const u64 *__restrict__ Al,
+                                                      const i64 *__restrict__ Ase, const u64 *__restrict__ xl,
+                                                      const i64 *__restrict__ xse, u64 *__restrict__ yl,
+                                                      i64 *__restrict__ yse)
 {
   static_assert (FG_G >= 4, "precision beyond the cooperative frame");
+  namespace cd = mpgfx::coopd;
+  constexpr int G = FG_G, R = FG_R, REC = NL + 1, BUF = R * G * REC;
+  __shared__ u64 lds[2 * BUF];
+  const int lane = (int) (threadIdx.x & 63), r = lane / G, j = lane % G;
+  const i64 i = (i64) blockIdx.x * R + r, ii = i < m ? i : m - 1;
+  const i64 nA = m * k;
+  if (threadIdx.x >= 64)
     {
-      namespace cd = mpgfx::coopd;
-      constexpr int G = FG_G, R = FG_R, REC = NL + 1;
-      __shared__ u64 lds[R * G * REC];
-      const cd::DevLanes w;
-      const int lane = (int) threadIdx.x, r = lane / G, j = lane % G;
-      const int jj = j < 1 ? 0 : (j > NL ? NL - 1 : j - 1);
-      const i64 i = (i64) blockIdx.x * R + r, ii = i < m ? i : m - 1;
-      const i64 nA = m * k;
-      cd::cv acc = cd::zero ();
-      /* operands of this lane's first product */
+      /* producer: block b's products into buffer b & 1, then a barrier */
       i64 kk = j < k ? j : k - 1;
       cu_freal<PB> a = ld<PB> (Al, Ase, nA, ii * k + kk), xv = ldc<PB> (xl, xse, k, kk);
+      int buf = 0;
       for (i64 s0 = 0; s0 < k; s0 += G)
         {
           const cu_freal<PB> p = cu_fp::cu_fmul<PB> (a, xv);
-          u64 *rec = lds + (r * G + j) * REC;
-#pragma unroll
-          for (int l = 0; l < NL; l++) rec[l] = p.m[l];
-          rec[NL] = (u64) (((i64) p.exp << 1) | (p.sign < 0 ? 1 : 0));
-          __syncthreads ();
-          /* this block's records of the row (lane j' holds limb j'-1) */
-          cd::raw q[G];
-#pragma unroll
-          for (int d = 0; d < G; d++)
-            {
-              const u64 *src = lds + (r * G + d) * REC;
-              q[d].m = src[jj];
-              q[d].w = (i64) src[NL];
-            }
-          /* the next block's operands, loaded while this block folds */
+          /* the next block's operands, loaded while this one is stored */
           const i64 sn = s0 + G + j;
           kk = sn < k ? sn : k - 1;
           a = ld<PB> (Al, Ase, nA, ii * k + kk);
           xv = ldc<PB> (xl, xse, k, kk);
+          u64 *rec = lds + buf * BUF + (r * G + j) * REC;
+#pragma unroll
+          for (int l = 0; l < NL; l++) rec[l] = p.m[l];
+          rec[NL] = (u64) (((i64) p.exp << 1) | (p.sign < 0 ? 1 : 0));
+          __syncthreads ();
+          buf ^= 1;
+        }
+      __syncthreads ();
+    }
+  else
+    {
+      /* folder: lane j of a row holds limb j - 1 */
+      const cd::DevLanes w;
+      const int jj = j < 1 ? 0 : (j > NL ? NL - 1 : j - 1);
+      cd::cv acc = cd::zero ();
+      int buf = 0;
+      __syncthreads ();
+      for (i64 s0 = 0; s0 < k; s0 += G)
+        {
+          cd::raw q[G];
+#pragma unroll
+          for (int d = 0; d < G; d++)
+            {
+              const u64 *src = lds + buf * BUF + (r * G + d) * REC;
+              q[d].m = src[jj];
+              q[d].w = (i64) src[NL];
+            }
+          __syncthreads ();
+          buf ^= 1;
           const i64 steps = k - s0;
 #pragma unroll
           for (int d = 0; d < G; d++)
@@ -919,7 +938,6 @@ __global__ __launch_bounds__ (64) void k_gemv_fused (i64 m, i64 k, const u64 *__
               acc.exp = live ? nv.exp : acc.exp;
               acc.sign = live ? nv.sign : acc.sign;
             }
-          __syncthreads ();
         }
       cval out;
       out.m = acc.m;
@@ -931,8 +949,11 @@ __global__ __launch_bounds__ (64) void k_gemv_fused (i64 m, i64 k, const u64 *__
 
 /* Complex GEMV y_i = fold_k (acc + RNDN_c(A[i,k] x[k])), per-op in k order
  * (each product component rounded once, cu_cmul; the complex addition
- * componentwise, cu_cadd), with the products inside the cooperative fold as in
- * k_gemv_fused.  Complex addition is componentwise, so the real and the
+ * componentwise, cu_cadd), with the products inside the cooperative fold
+ * (the fold wavefront computes them itself: it holds two chains per row, two
+ * folders share a SIMD, and a separate producer wavefront, as in k_gemv_fused,
+ * measured slower, 6.5 against 5.6 ms at pb 512 4096^2).  Complex addition is
+ * componentwise, so the real and the
  * imaginary parts of a row are two independent real folds: a wavefront holds
  * 64/G chains, chain c the part c & 1 of row c >> 1.  Per block of G steps
  * every lane computes one product component, RNDN(ar u -/+ ai v) for its
@@ -1149,8 +1170,8 @@ int gemv (i64 m, i64 k, const u64 *Al, const i64 *Ase, const u64 *xl, const i64 
       return launch_status ("gemv_real");
     }
   /* products inside the fold: no scratch */
-  hipLaunchKernelGGL (k_gemv_fused, dim3 ((unsigned) ((m + FG_R - 1) / FG_R)), dim3 (64), 0, s, m, k, Al, Ase, xl, xse,
-                      yl, yse);
+  hipLaunchKernelGGL (k_gemv_fused, dim3 ((unsigned) ((m + FG_R - 1) / FG_R)), dim3 (128), 0, s, m, k, Al, Ase, xl,
+                      xse, yl, yse);
   return launch_status ("gemv_real");
 }
 int pack (int cplx, i64 n, const void *aos, u64 *l, i64 *se, hipStream_t s)
