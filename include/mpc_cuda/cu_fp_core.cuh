@@ -239,9 +239,43 @@ struct short_cut
   static constexpr int ERR = C::LB * K0 + C::LB + lg (K0) + 2 - LSB;
 };
 
+/* W < 16: the short product by 32-bit word columns.  Columns K0 = 2W - H - 1
+ * and up are computed (one column below the kept words, whose carries it
+ * passes on exactly); the columns below K0 would add less than
+ * sum_{k<K0} (k+1) 2^(64+32k) < (K0+1) 2^(32K0+32), so the result is a lower
+ * bound of the top H words off by less than K0 + 1 units of its last word. */
+template <int W, int H>
+struct short_cols
+{
+  static_assert (H <= 2 * W - 1, "short product wider than the product");
+  static constexpr int K0 = 2 * W - H - 1;
+  static constexpr int lg (int x) { return x <= 1 ? 0 : 1 + lg ((x + 1) / 2); }
+  static constexpr int ERR = lg (K0 + 1);
+};
+
+/* the error exponent of mul_words_short<W, H> (either form) */
+template <int W, int H>
+constexpr int short_err ()
+{
+  if constexpr (W >= 16)
+    return short_cut<W, H>::ERR;
+  else
+    return short_cols<W, H>::ERR;
+}
+
+template <int W, int H>
+CUFP_INL void mul_words_short_cols (u32 (&p)[H], const u32 (&a)[W], const u32 (&b)[W]);
+
 template <int W, int H>
 CUFP_INL void mul_words_short (u32 (&p)[H], const u32 (&a)[W], const u32 (&b)[W])
 {
+  if constexpr (W < 16)
+    {
+      mul_words_short_cols<W, H> (p, a, b);
+      return;
+    }
+  else
+    {
   typedef limb_cut<W> C;
   constexpr int NL = C::NL, LB = C::LB, K0 = short_cut<W, H>::K0;
   u32 La[NL], Lb[NL], P[2 * NL];
@@ -265,6 +299,7 @@ CUFP_INL void mul_words_short (u32 (&p)[H], const u32 (&a)[W], const u32 (&b)[W]
     static_assert (k0 >= K0, "word below the computed columns");
     p[decltype (Q)::value] = limbs_word<LB, 2 * NL, k0, s0> (P);
   });
+    }
 }
 
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -304,6 +339,31 @@ __device__ __forceinline__ void mul_words_cols (u32 (&p)[2 * W], const u32 (&a)[
   p[2 * W - 1] = (u32) acc;
 }
 #endif
+
+template <int W, int H>
+CUFP_INL void mul_words_short_cols (u32 (&p)[H], const u32 (&a)[W], const u32 (&b)[W])
+{
+  constexpr int K0 = short_cols<W, H>::K0, LO = 2 * W - H;
+  u64 acc = 0;
+  sfor<K0, 2 * W - 1> ([&] (auto K) {
+    constexpr int k = decltype (K)::value;
+    constexpr int i0 = k >= W ? k - W + 1 : 0, i1 = k < W ? k + 1 : W;
+    u32 ovf = 0;
+    sfor<i0, i1> ([&] (auto I) {
+      constexpr int i = decltype (I)::value;
+#if defined(__HIP_DEVICE_COMPILE__)
+      mac_carry (acc, ovf, a[i], b[k - i]);
+#else
+      const u64 s = acc + (u64) a[i] * b[k - i];
+      ovf += s < acc ? 1u : 0u;
+      acc = s;
+#endif
+    });
+    if constexpr (k >= LO) p[k - LO] = (u32) acc;
+    acc = (acc >> 32) | ((u64) ovf << 32);
+  });
+  p[H - 1] = (u32) acc;
+}
 
 template <int W>
 CUFP_INL void mul_words (u32 (&p)[2 * W], const u32 (&a)[W], const u32 (&b)[W])

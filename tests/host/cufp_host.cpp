@@ -111,7 +111,7 @@ template <int PB> void fmm_chk (int64_t n, const uint64_t *l, const int64_t *se,
       amb[i] = f ? 1 : 0;
     }
 }
-#define PB_SHORT(X) X(512) X(1024) X(1056) X(2048)
+#define PB_SHORT(X) X(256) X(288) X(320) X(384) X(512) X(1024) X(1056) X(2048)
 extern "C" int host_fmm_check (int pb, int64_t n, const uint64_t *l, const int64_t *se, int sub, uint64_t *sl,
                                int64_t *sse, uint64_t *el, int64_t *ese, uint8_t *amb)
 {

@@ -1,7 +1,7 @@
 """The short-product paths -- of the complex product component RNDN(a*b -/+
 c*d) (core::fmm_short, include/mpc_cuda/cu_fcomplex.cuh) and of the real
-product RNDN(a*b) (core::fmul_short, cu_freal.cuh), used for the limb-product
-precisions -- against the exact paths (core::fmm_exact, core::fmul_exact), both from
+product RNDN(a*b) (core::fmul_short, cu_freal.cuh), by word columns below 16
+words and by limb columns above -- against the exact paths (core::fmm_exact, core::fmul_exact), both from
 the g++ build of the headers (tests/host/libcufp_host.so): wherever the short
 path does not flag a lane as undecided, its result is the exact one, bit for
 bit; the cases its error bound cannot decide (exact and massive cancellation,
@@ -18,7 +18,7 @@ from mpc_cuda_amd.arrays import nlimbs
 from mpcases import pack
 
 HOSTLIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "host", "libcufp_host.so")
-PBS = [512, 1024, 1056, 2048]
+PBS = [256, 288, 320, 384, 512, 1024, 1056, 2048]
 
 
 @pytest.fixture(scope="module")
