@@ -150,6 +150,45 @@ def test_exact_async_fixup_of_undecided_elements(gpu):
     assert O.mismatches(ra.pair(), ref, ignore_zero_sign=False) == 0
 
 
+def test_exact_async_fixup_in_graph_replays(gpu):
+    """The fix-up of undecided elements inside a captured HIP graph: the
+    planted rows (ties, near-ties, exact zeros) captured once and replayed
+    three times, with the output overwritten between the replays (an eager
+    fill and a synchronisation); every replay equals the eager call and the
+    oracle."""
+    import torch
+    from mpcases import pack
+    pb, m, n, k = 256, 20, 8, 64
+    Av, Bv = _planted_undecided(pb, m, n, k, 131)
+    A2, B2 = pack(Av, pb), pack(Bv, pb)
+    A, B = A2.cuda(), B2.cuda()
+    C = M.empty(pb, m * n)
+    ws = torch.empty(M.gemm_workspace_size(pb, "exact", m, n, k), dtype=torch.uint8, device="cuda")
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        M.gemm(A, B, m, n, k, "exact", out=C, workspace=ws)       # warm-up outside the capture
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        M.gemm(A, B, m, n, k, "exact", out=C, workspace=ws)
+    torch.cuda.synchronize()
+    want = M.gemm(A, B, m, n, k, "exact")
+    ref = O.gemm(pb, m, n, k, A2.pair(), B2.pair(), "exact")
+    torch.cuda.synchronize()
+    for r in range(3):
+        C.limbs.fill_(0x5A5A + r)
+        C.se.fill_(12345 + r)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert int(ws[:8].view(torch.int64).item()) == _lib.OK
+        assert int(ws[60:64].view(torch.int32).item()) > 0
+        assert M.equal(C, want) == 0, r
+        assert O.mismatches(C.pair(), ref, ignore_zero_sign=False) == 0, r
+
+
 def test_complex_exact_fixup_of_undecided_elements(gpu):
     """The complex exact GEMM (two real exact GEMMs over 2k terms in the
     asynchronous mode) on the planted rows as real parts, with zero
